@@ -1,0 +1,65 @@
+// hss_kernels.h -- launch interface between the host runtime and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hss_types.h"
+
+namespace hss {
+
+struct BuildBuffers {
+    uint64_t n;
+    uint32_t* spf;                // [n+1]
+    uint32_t* H;                  // [n+2]
+    const uint32_t* small_primes;  // primes <= sqrt(n)
+    uint32_t nsmall;
+};
+
+struct CellParams {
+    // context tables
+    const uint32_t* off;
+    const double* cre;
+    const double* cim;
+    const uint32_t* cres;  // (re mod 180) | (im mod 180) << 16
+    // batch
+    const PosDesc* pos;
+    const LeftDesc* lefts;
+    const TileDesc* tiles;
+    uint32_t ntiles;
+    const LeftFast* lfast;
+    const i128* lre;
+    const i128* lim;
+    uint32_t* next_tile;
+    unsigned long long* cand;    // per position
+    unsigned long long* leaves;  // per position
+    unsigned long long* pairs;   // per position: pairs reaching the enumeration
+    DefItem* def;     // leaves for the exact kernel
+    uint32_t* ndef;
+    uint32_t def_cap;
+    HitOut* raw;      // exact fourth powers before the acceptance rules
+    uint32_t* nraw;
+    uint32_t raw_cap;
+    uint64_t limit;
+    double n4_d;
+    QR4 qr;
+};
+
+hipError_t launch_build_tables(const BuildBuffers& B, hipStream_t st);
+hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t* sums, uint64_t n, hipStream_t st);
+hipError_t launch_collect_split(const uint32_t* spf, uint64_t n, uint32_t* plist, uint32_t* psize,
+                                uint32_t* count, hipStream_t st);
+hipError_t launch_prime_values(const uint32_t* plist, uint32_t np, const uint32_t* vstart, uint32_t total,
+                               uint32_t* vbase, int64_t* pa, int64_t* pb, i128* vre, i128* vim, hipStream_t st);
+hipError_t launch_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
+                             const uint32_t* vbase, const i128* vre, const i128* vim, double* cre, double* cim,
+                             uint32_t* cres, hipStream_t st);
+hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uint32_t nleft, const uint32_t* vbase,
+                             const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast,
+                             hipStream_t st);
+hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st);
+hipError_t launch_exact(const CellParams& P, uint32_t n, hipStream_t st);
+hipError_t launch_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+                         uint32_t* nhits, uint32_t hit_cap, hipStream_t st);
+hipError_t cell_occupancy(int* blocks_per_cu);
+
+}  // namespace hss

@@ -1,0 +1,560 @@
+// hss_kernels.hip -- gfx950 kernels of the Gaussian-integer enumeration core.
+//
+// Device-resident tables (built once per context for every integer <= N_max):
+//   spf[i]            smallest prime factor (on-device sieve)
+//   off[i]            start of i's half-set in the Gaussian cache (exclusive scan
+//                     of H_i = (w(i)+1)/2, w(i) = prod(4e+1) over p = 1 (mod 4))
+//   cre/cim/cres      the cache: for every i, the (w(i)+1)/2 Gaussian integers
+//                     fixed(i) * prod pi_p^r conj(pi_p)^(4e_p - r) whose digit
+//                     tuples are lexicographically <= the central tuple; FP64
+//                     coordinates (exact: |coord| <= i^2 < 2^53) and residues mod
+//                     Q1.  The last entry of each half-set is the real one.
+//   pval              per split prime p and merged exponent E <= 2*floor(log_p N):
+//                     pi^r conj(pi)^(4E-r), r = 0..4E, exact i128
+//
+// Reference counterparts: Search::prepare, prepare_gaussian_primes,
+// flatten_prime_data, prepare_factor_sets (src/gpu_search_v42.cu:354-391,
+// 525-607; src/search_v18.cpp:159-197, 285-309).  The reference stores
+// free/central/raw lists per integer (3w+1)/2 entries; this cache keeps only
+// the half-set (w+1)/2 and takes conjugates on the fly.
+//
+// The search kernel (cell_kernel) replaces search_kernel
+// (src/gpu_search_v42.cu:292-345): a persistent grid pulls tiles
+// (left set x range of partner indices) from a device work queue; each
+// wavefront maps one partner half-set entry v per lane and walks the left set
+// u uniformly (scalar loads), evaluating the cells u*v and u*conj(v) with the
+// FP64/residue fast path of hss_math.h, which decides every candidate count
+// exactly.  Counted terms whose zz could be a fourth power are compacted with
+// ballot/mbcnt into a per-wave LDS queue and root-tested 64 at a time; the few
+// survivors, and leaves left undecided by the FP64 band, go to a global buffer
+// that the exact kernel (k_exact) settles with 128/256-bit integer arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "hss_build.h"
+#include "hss_kernels.h"
+
+namespace hss {
+
+// ------------------------------------------------------------- table build
+
+__global__ void k_fill_identity(uint32_t* spf, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) spf[i] = (uint32_t)i;
+}
+
+// block b marks the multiples j >= p^2 of the b-th small prime p
+__global__ void k_sieve_mark(uint32_t* spf, const uint32_t* primes, uint64_t n) {
+    uint64_t p = primes[blockIdx.x];
+    for (uint64_t j = p * p + (uint64_t)threadIdx.x * p; j <= n; j += (uint64_t)blockDim.x * p)
+        atomicMin(&spf[j], (uint32_t)p);
+}
+
+// H_i = (w(i)+1)/2 for i <= n, H_0 = 0 and H_{n+1} = 0 (scan sentinel)
+__global__ void k_half_counts(const uint32_t* spf, uint32_t* H, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n + 1) return;
+    if (i == 0 || i == n + 1) {
+        H[i] = 0;
+        return;
+    }
+    uint32_t x = (uint32_t)i, w = 1;
+    while (x > 1) {
+        uint32_t p = spf[x], e = 0;
+        while (x % p == 0) {
+            x /= p;
+            ++e;
+        }
+        if ((p & 3) == 1) w *= 4 * e + 1;
+    }
+    H[i] = (w + 1) / 2;
+}
+
+// exclusive scan of n u32 values, 4096 per block (256 threads x 16)
+__global__ void k_scan_blocks(const uint32_t* in, uint32_t* out, uint32_t* block_sums, uint64_t n) {
+    __shared__ uint32_t s[256];
+    uint64_t base = (uint64_t)blockIdx.x * 4096 + threadIdx.x * 16;
+    uint32_t v[16], sum = 0;
+    for (int j = 0; j < 16; ++j) {
+        v[j] = base + j < n ? in[base + j] : 0;
+        sum += v[j];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (int j = 0; j < 16; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 255) block_sums[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_sums(uint32_t* sums, uint32_t nblocks) {
+    __shared__ uint32_t s[1024];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += 1024) {
+        uint32_t i = b0 + threadIdx.x;
+        uint32_t v = i < nblocks ? sums[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            uint32_t t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks) sums[i] = carry + s[threadIdx.x] - v;
+        uint32_t tot = s[1023];
+        __syncthreads();
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_add(uint32_t* out, const uint32_t* sums, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += sums[i / 4096];
+}
+
+// collect the split primes p = 1 (mod 4), p <= n (unordered) with their
+// value-block sizes
+__global__ void k_collect_split(const uint32_t* spf, uint64_t n, uint32_t* plist, uint32_t* psize,
+                                uint32_t* count) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t p = 4 * i + 5;
+    if (p > n || spf[p] != p) return;
+    uint32_t a = 0;
+    for (unsigned __int128 q = p; q <= n; q *= p) ++a;
+    uint32_t Emax = 2 * a;
+    uint32_t slot = atomicAdd(count, 1u);
+    plist[slot] = (uint32_t)p;
+    psize[slot] = Emax * (2 * Emax + 3);
+}
+
+// Gaussian split pi = a + bi of each collected prime by Cornacchia's method
+__global__ void k_prime_split(const uint32_t* plist, uint32_t np, const uint32_t* vstart, uint32_t* vbase,
+                              int64_t* pa, int64_t* pb) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    uint64_t p = plist[t];
+    int64_t a, b;
+    prime_split(p, a, b);
+    pa[t] = a;
+    pb[t] = b;
+    vbase[p] = vstart[t];
+}
+
+// value v = pi^r conj(pi)^(4E-r) for every (prime, E, r)
+__global__ void k_prime_values(const uint32_t* plist, uint32_t np, const uint32_t* vstart, uint32_t total,
+                               const int64_t* pa, const int64_t* pb, i128* vre, i128* vim) {
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    uint32_t lo = 0, hi = np;  // last t with vstart[t] <= idx
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (vstart[mid] <= idx)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint32_t off = idx - vstart[lo];
+    uint32_t E = 1;
+    while (exp_offset(E + 1) <= off) ++E;
+    uint32_t r = off - exp_offset(E);
+    GInt v = gauss_value(pa[lo], pb[lo], E, r);
+    vre[idx] = v.re;
+    vim[idx] = v.im;
+    (void)plist;
+}
+
+// one cache entry per thread
+__global__ void k_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
+                             const uint32_t* vbase, const i128* vre, const i128* vim, double* cre, double* cim,
+                             uint32_t* cres) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    uint64_t lo = 0, hi = n + 1;  // last i with off[i] <= e
+    while (hi - lo > 1) {
+        uint64_t mid = (lo + hi) / 2;
+        if (off[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint32_t i = (uint32_t)lo;
+    uint32_t t = e - off[i];
+    uint32_t ps[MAX_SPLIT], es[MAX_SPLIT], k = 0;
+    int64_t fixed = 1;
+    uint32_t x = i;
+    while (x > 1) {
+        uint32_t p = spf[x], a = 0;
+        while (x % p == 0) {
+            x /= p;
+            ++a;
+        }
+        if ((p & 3) == 1) {
+            ps[k] = p;
+            es[k] = a;
+            ++k;
+        } else {
+            for (uint32_t j = 0; j < 2 * a; ++j) fixed *= p;
+        }
+    }
+    uint32_t dig[MAX_SPLIT];
+    decode_rank(t, k, es, dig);
+    int64_t re = fixed, im = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        uint32_t at = vbase[ps[j]] + exp_offset(es[j]) + dig[j];
+        int64_t a = (int64_t)vre[at], b = (int64_t)vim[at];
+        int64_t nr = re * a - im * b, ni = re * b + im * a;
+        re = nr;
+        im = ni;
+    }
+    cre[e] = (double)re;
+    cim[e] = (double)im;
+    int64_t r1 = re % (int64_t)QA, i1 = im % (int64_t)QA;
+    cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
+}
+
+// one block per left set
+__global__ void k_build_left(const LeftBuild* builds, const LeftDesc* lefts, const uint32_t* vbase,
+                             const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast) {
+    const LeftBuild b = builds[blockIdx.x];
+    const LeftDesc L = lefts[blockIdx.x];
+    for (uint32_t t0 = threadIdx.x; t0 < L.H; t0 += blockDim.x) {
+        uint32_t dig[MAX_SPLIT];
+        decode_rank(t0, b.k, b.E, dig);
+        GInt z = {(i128)b.fixed, 0};
+        for (uint32_t j = 0; j < b.k; ++j) {
+            uint32_t at = vbase[b.p[j]] + exp_offset(b.E[j]) + dig[j];
+            z = gmul(z, GInt{vre[at], vim[at]});
+        }
+        uint32_t e = L.off + t0;
+        lre[e] = z.re;
+        lim[e] = z.im;
+        LeftFast f;
+        f.re = i128_to_double(z.re);
+        f.im = i128_to_double(z.im);
+        f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
+        f.pad = 0;
+        lfast[e] = f;
+    }
+}
+
+// ------------------------------------------------------------ cell kernel
+
+static constexpr int WG = 256;
+static constexpr int WAVES = WG / 64;
+static constexpr int QCAP = 128;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
+    for (int s = 1; s < 64; s <<= 1) {
+        uint32_t t = __shfl_up(v, (unsigned)s, 64);
+        if (lane >= s) v += t;
+    }
+    return v;
+}
+
+__device__ inline uint64_t wave_sum64(uint64_t v) {
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    return v;
+}
+
+__device__ inline uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline bool divides(uint32_t k, uint32_t inv, uint32_t thr) { return k * inv <= thr; }
+
+// Append the lanes with pred set to the global deferred buffer.
+__device__ inline void push_global(const CellParams& P, bool pred, const DefItem& it) {
+    uint64_t mask = __ballot(pred);
+    if (!mask) return;
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(P.ndef, (uint32_t)__popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+    uint32_t slot = base + lane_rank(mask);
+    if (pred && slot < P.def_cap) P.def[slot] = it;
+}
+
+// Per-wave LDS queue of counted terms awaiting the root-proximity test.
+struct RootQueue {
+    double* z;
+    uint4* id;  // (ui, vidx, k | conj << 30, unused)
+    uint32_t n;
+};
+
+__device__ inline void root_drain(const CellParams& P, RootQueue& q, uint32_t left, uint32_t count) {
+    const uint32_t lane = __lane_id();
+    bool pass = false;
+    uint4 id = make_uint4(0, 0, 0, 0);
+    if (lane < count) {
+        id = q.id[lane];
+        pass = root_near_integer(q.z[lane]);
+    }
+    push_global(P, pass, DefItem{left, id.x, id.y, id.z});
+}
+
+__device__ inline void root_push(const CellParams& P, RootQueue& q, uint32_t left, bool pred, double zz,
+                                 uint32_t ui, uint32_t vidx, uint32_t kc) {
+    uint64_t mask = __ballot(pred);
+    if (!mask) return;
+    uint32_t slot = q.n + lane_rank(mask);
+    if (pred) {
+        q.z[slot] = zz;
+        q.id[slot] = make_uint4(ui, vidx, kc, 0);
+    }
+    q.n += (uint32_t)__popcll(mask);
+    if (q.n >= 64) {
+        __builtin_amdgcn_wave_barrier();
+        root_drain(P, q, left, 64);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t lane = __lane_id();
+        double mz = q.z[64 + lane];
+        uint4 mi = q.id[64 + lane];
+        __builtin_amdgcn_wave_barrier();
+        if (lane + 64 < q.n) {
+            q.z[lane] = mz;
+            q.id[lane] = mi;
+        }
+        __builtin_amdgcn_wave_barrier();
+        q.n -= 64;
+    }
+}
+
+__global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
+    __shared__ double s_qz[WAVES][QCAP];
+    __shared__ uint4 s_qid[WAVES][QCAP];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    RootQueue rq{s_qz[wave], s_qid[wave], 0};
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(P.next_tile, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        if (t >= P.ntiles) break;
+        const TileDesc T = P.tiles[t];
+        const LeftDesc L = P.lefts[T.left];
+        const PosDesc& S = P.pos[L.pos];
+        uint64_t cand = 0, leaves = 0, pairs = 0;
+        for (uint32_t par = 0; par < 2; ++par) {
+            uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
+            for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
+                uint32_t k = kb + 2u * (uint32_t)lane;
+                uint32_t Hk = 0;
+                if (k <= T.k1) {
+                    bool ok = !(S.l_even && !(k & 1u));
+                    for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
+                    for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
+                    if (ok) {
+                        Hk = P.off[k + 1] - P.off[k];
+                        if (L.trivial && Hk == 1) Hk = 0;
+                    }
+                }
+                pairs += Hk ? 1 : 0;
+                uint32_t incl = wave_incl_scan(Hk, lane);
+                uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+                for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                    uint32_t e = e0 + (uint32_t)lane;
+                    bool act = e < total;
+                    uint32_t j = 0;
+                    for (uint32_t s = 32; s > 0; s >>= 1) {
+                        uint32_t v = __shfl(incl, (int)(j + s - 1), 64);
+                        if (v <= e) j += s;
+                    }
+                    j = j > 63 ? 63 : j;
+                    uint32_t inclj = __shfl(incl, (int)j, 64);
+                    uint32_t Hj = __shfl(Hk, (int)j, 64);
+                    uint32_t kj = kb + 2u * j;
+                    uint32_t jj = e - (inclj - Hj);
+                    uint32_t vidx = act ? P.off[kj] + jj : 0u;
+                    bool vc = jj + 1 == Hj;
+                    double c = P.cre[vidx], d = P.cim[vidx];
+                    uint32_t vr = P.cres[vidx];
+                    uint32_t c180 = vr & 0xFFFFu, d180 = vr >> 16;
+                    uint64_t m = (uint64_t)L.q * kj;
+                    uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
+                    LaneCtx ctx;
+                    make_lane_ctx(m, S.l4_d, S.l2_d, P.n4_d, S.l3, S.l5, m3, m5, ctx);
+                    if (act) leaves += vc ? L.H : 2ull * L.H - 1;
+                    for (uint32_t ui = 0; ui < L.H; ++ui) {
+                        const LeftFast u = P.lfast[L.off + ui];
+                        const bool uc = ui + 1 == L.H;
+                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                        const double t1 = u.im * d, t2 = u.re * d;
+                        const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
+                        for (uint32_t cj = 0; cj < 2; ++cj) {
+                            if (cj == 1 && uc) break;
+                            const bool live = act && (cj == 0 || !vc);
+                            double X, Y;
+                            uint32_t vX, vY;
+                            if (cj == 0) {
+                                X = __builtin_fma(u.re, c, -t1);
+                                Y = __builtin_fma(u.im, c, t2);
+                                vX = pac + QA * QA - pbd;
+                                vY = pad + pbc;
+                            } else {
+                                X = __builtin_fma(u.re, c, t1);
+                                Y = __builtin_fma(u.im, c, -t2);
+                                vX = pac + pbd;
+                                vY = pbc + QA * QA - pad;
+                            }
+
+                            LeafEval r;
+                            if (cj == 0 && uc && vc)
+                                eval_real_leaf(vX, ctx, r);  // both factors real
+                            else
+                                eval_leaf(X, Y, vX, vY, ctx, r);
+                            if (live && !r.und) cand += r.cnt;
+                            const uint32_t kc = kj | cj << 30;
+                            push_global(P, live && r.und, DefItem{T.left, ui, vidx, kc | 0x80000000u});
+                            const bool ok = live && !r.und;
+                            for (int tm = 0; tm < 4; ++tm) root_push(P, rq, T.left, ok && r.f[tm], r.zz[tm], ui, vidx, kc);
+                        }
+                    }
+                }
+            }
+        }
+        if (rq.n) {
+            __builtin_amdgcn_wave_barrier();
+            root_drain(P, rq, T.left, rq.n);
+            rq.n = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        cand = wave_sum64(cand);
+        leaves = wave_sum64(leaves);
+        pairs = wave_sum64(pairs);
+        if (lane == 0) {
+            atomicAdd(P.cand + L.pos, (unsigned long long)cand);
+            atomicAdd(P.leaves + L.pos, (unsigned long long)leaves);
+            atomicAdd(P.pairs + L.pos, (unsigned long long)pairs);
+        }
+    }
+}
+
+// Exact settlement of deferred leaves: full reference chain with 128-bit
+// integers; counts are added only for leaves the fast path left undecided.
+__global__ void k_exact(const CellParams P, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const DefItem it = P.def[i];
+        const LeftDesc L = P.lefts[it.left];
+        const PosDesc& S = P.pos[L.pos];
+        const uint32_t k = it.kflags & 0x3FFFFFFFu;
+        const bool conj = (it.kflags >> 30) & 1u, full = it.kflags >> 31;
+        const uint64_t m = (uint64_t)L.q * k;
+        ExactCtx e;
+        make_exact_ctx(S.l, m, P.limit, e);
+        i128 zre, zim;
+        exact_product(P.lre[L.off + it.ui], P.lim[L.off + it.ui], (int64_t)P.cre[it.vidx], (int64_t)P.cim[it.vidx],
+                      conj, zre, zim);
+        uint32_t c = exact_leaf(zre, zim, e, P.qr, [&](uint64_t r, u128 a4) {
+            uint32_t slot = atomicAdd(P.nraw, 1u);
+            if (slot < P.raw_cap) P.raw[slot] = HitOut{S.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64)};
+        });
+        if (full && c) atomicAdd(P.cand + L.pos, (unsigned long long)c);
+    }
+}
+
+// Acceptance of the raw exact fourth powers: gcd, 5 | lmn, ordering and the
+// 256-bit Heron recheck; accepted triples are appended to the hit list.
+__global__ void k_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+                         uint32_t* nhits, uint32_t hit_cap) {
+    uint32_t n = *nraw < raw_cap ? *nraw : raw_cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        HitOut h = raw[i];
+        u128 a4 = ((u128)h.area4_hi << 64) | h.area4_lo;
+        if (!accept_hit(h.l, h.m, h.n, limit, a4)) continue;
+        uint32_t slot = atomicAdd(nhits, 1u);
+        if (slot < hit_cap) hits[slot] = h;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+#define HSS_CHECK(x)                                  \
+    do {                                              \
+        hipError_t err_ = (x);                        \
+        if (err_ != hipSuccess) return err_;          \
+    } while (0)
+
+hipError_t launch_build_tables(const BuildBuffers& B, hipStream_t st) {
+    uint64_t n = B.n;
+    uint32_t g = (uint32_t)((n + 256) / 256);
+    hipLaunchKernelGGL(k_fill_identity, dim3(g), dim3(256), 0, st, B.spf, n);
+    if (B.nsmall) hipLaunchKernelGGL(k_sieve_mark, dim3(B.nsmall), dim3(1024), 0, st, B.spf, B.small_primes, n);
+    uint32_t g2 = (uint32_t)((n + 2 + 255) / 256);
+    hipLaunchKernelGGL(k_half_counts, dim3(g2), dim3(256), 0, st, (const uint32_t*)B.spf, B.H, n);
+    HSS_CHECK(hipGetLastError());
+    return hipSuccess;
+}
+
+hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint32_t* sums, uint64_t n, hipStream_t st) {
+    uint32_t nb = (uint32_t)((n + 4095) / 4096);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, in, out, sums, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, out, (const uint32_t*)sums, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect_split(const uint32_t* spf, uint64_t n, uint32_t* plist, uint32_t* psize,
+                                uint32_t* count, hipStream_t st) {
+    uint64_t cnt = n >= 5 ? (n - 5) / 4 + 1 : 0;
+    if (!cnt) return hipSuccess;
+    hipLaunchKernelGGL(k_collect_split, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, spf, n, plist, psize,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t launch_prime_values(const uint32_t* plist, uint32_t np, const uint32_t* vstart, uint32_t total,
+                               uint32_t* vbase, int64_t* pa, int64_t* pb, i128* vre, i128* vim, hipStream_t st) {
+    if (!np) return hipSuccess;
+    hipLaunchKernelGGL(k_prime_split, dim3((np + 255) / 256), dim3(256), 0, st, plist, np, vstart, vbase, pa, pb);
+    hipLaunchKernelGGL(k_prime_values, dim3((total + 255) / 256), dim3(256), 0, st, plist, np, vstart, total,
+                       (const int64_t*)pa, (const int64_t*)pb, vre, vim);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
+                             const uint32_t* vbase, const i128* vre, const i128* vim, double* cre, double* cim,
+                             uint32_t* cres, hipStream_t st) {
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_cache, dim3((total + 255) / 256), dim3(256), 0, st, spf, off, n, total, vbase, vre,
+                       vim, cre, cim, cres);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uint32_t nleft, const uint32_t* vbase,
+                             const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast,
+                             hipStream_t st) {
+    if (!nleft) return hipSuccess;
+    hipLaunchKernelGGL(k_build_left, dim3(nleft), dim3(256), 0, st, builds, lefts, vbase, vre, vim, lre, lim, lfast);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(cell_kernel, dim3(blocks), dim3(WG), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact(const CellParams& P, uint32_t n, hipStream_t st) {
+    if (!n) return hipSuccess;
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_exact, dim3(blocks), dim3(256), 0, st, P, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+                         uint32_t* nhits, uint32_t hit_cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_accept, dim3(16), dim3(256), 0, st, raw, nraw, raw_cap, limit, hits, nhits, hit_cap);
+    return hipGetLastError();
+}
+
+hipError_t cell_occupancy(int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, cell_kernel, WG, 0);
+}
+
+}  // namespace hss

@@ -1,0 +1,360 @@
+// hss_math.h -- arithmetic shared by the gfx950 kernels and the host-side checker.
+//
+// Problem recap (docs/ALGORITHM.en.md:25-107 of the reference).  For root sides
+// l <= m and LIMIT N every Gaussian integer z = D + 4A*i of norm (2 l^2 m^2)^2 is
+// enumerated once per conjugation class; for each orientation (d, area4) in
+// {(Re z, Im z), (Im z, Re z) unless Re z = +-Im z} and each sign of d the
+// reference (src/search_v18.cpp:436-479) counts a "candidate" when
+//     area4 != 0, area4 = 0 (mod 8), lo <= d <= hi  (lo = l^4+m^4-N^4, hi = l^4),
+//     |area4| = 0 (mod 3), 3 | lm  => |area4| = 0 (mod 9),
+//     5 divides neither l nor m   => |area4| = 0 (mod 5),
+// (the negative sign only when d != 0), and then accepts the triple when
+// zz = l^4 + m^4 - d passes the fourth-power residue tests modulo 16, 625, 9, 7,
+// 13, 17, is an exact fourth power n^4, gcd(l, m, n) = 1, 5 | l*m*n and
+// m <= n <= N.
+//
+// Fast path.  Every z is 2*u*v (or 2*u*conj(v)) with u from a "left set" of
+// the root l and v from the half-set of a partner k; u and v carry FP64
+// coordinates and exact residues modulo 180 = 4 * 9 * 5.  Writing z/2 = X + iY:
+//   * the area tests need X or Y modulo 4 and modulo h = (3 or 9)*(1 or 5),
+//     decided exactly from the residues;
+//   * the range test is decided from FP64 estimates whose error is provably
+//     below tau/16; an estimate within tau of a bound is "undecided" and the
+//     whole leaf goes to the exact stage;
+//   * a counted term's zz is estimated in FP64 (relative error < 2^-49 because
+//     zz >= m^4); unless its fourth root lies within ROOT_EPS of an integer it
+//     cannot be n^4, so only those terms reach the exact stage.
+// The exact stage (exact_leaf) restates the reference chain with 128-bit
+// integers and is the only place where residues mod 16/625/9/7/13/17, the
+// integer fourth root and the acceptance rules are evaluated.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define HSS_HD __host__ __device__
+#else
+#define HSS_HD
+#endif
+
+namespace hss {
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+static constexpr uint32_t QA = 180;        // area residue modulus 4 * 9 * 5
+static constexpr double ROOT_EPS = 1e-4;   // root proximity band (errors < 1e-7)
+
+// ------------------------------------------------------------------ residues
+
+HSS_HD inline uint32_t two64_mod(uint32_t q) {
+    uint64_t r = ((uint64_t)1 << 32) % q;
+    return (uint32_t)((r * r) % q);
+}
+
+// residue of a non-negative value below 2^128 modulo a small modulus q
+HSS_HD inline uint32_t umod128(u128 v, uint32_t q) {
+    uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
+    uint64_t t = (hi % q) * two64_mod(q) + lo % q;
+    return (uint32_t)(t % q);
+}
+
+// residue in [0, q) of a signed 128-bit value
+HSS_HD inline uint32_t smod128(i128 v, uint32_t q) {
+    if (v >= 0) return umod128((u128)v, q);
+    uint32_t r = umod128((u128)(-v), q);
+    return r ? q - r : 0;
+}
+
+// fourth-power residue sets as bit masks: bit r set iff r = x^4 (mod q)
+struct QR4 {
+    uint32_t m625[20];
+    uint32_t m9, m7, m13, m17;
+};
+
+inline QR4 make_qr4() {
+    QR4 q{};
+    for (uint32_t x = 0; x < 625; ++x) {
+        uint32_t r = (x * x % 625) * (x * x % 625) % 625;
+        q.m625[r >> 5] |= 1u << (r & 31);
+    }
+    auto small = [](uint32_t m) {
+        uint32_t b = 0;
+        for (uint32_t x = 0; x < m; ++x) b |= 1u << ((x * x % m) * (x * x % m) % m);
+        return b;
+    };
+    q.m9 = small(9);
+    q.m7 = small(7);
+    q.m13 = small(13);
+    q.m17 = small(17);
+    return q;
+}
+
+HSS_HD inline bool bit_of(const uint32_t* words, uint32_t i) { return (words[i >> 5] >> (i & 31)) & 1u; }
+
+// ------------------------------------------------- per-(l, m) fast-path state
+
+struct LaneCtx {
+    // range bounds on |2D| widened (w) and narrowed (n) by tau:
+    //   term1: lo+ <= |2D| <= hi (d = +|2D|),  term2: |2D| <= -lo (d = -|2D|)
+    double t1_lo_w, t1_lo_n, t1_hi_w, t1_hi_n;
+    double t2_w, t2_n;
+    double tau;   // undecided band: FP64 errors stay below tau / 16
+    double lm;    // FP64 value of l^4 + m^4
+    uint32_t inv_h;  // inverse of h = (3 or 9) * (1 or 5) modulo 2^24
+    uint32_t thr_h;  // floor((2^24 - 1) / h)
+};
+
+HSS_HD inline uint32_t inv_mod_2_24(uint32_t h) {
+    uint32_t x = h;  // Newton iteration for the inverse of an odd h modulo 2^24
+    for (int i = 0; i < 5; ++i) x *= 2u - h * x;
+    return x & 0xFFFFFFu;
+}
+
+// Per-pair context.  l4_d, n4_d: FP64 values of l^4 and N^4; l2_d = l^2;
+// l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5.
+HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
+                                 uint32_t m3, uint32_t m5, LaneCtx& c) {
+    double m2 = (double)(m * m);  // exact: m < 2^26
+    double m4 = m2 * m2;
+    double lo = (l4_d + m4) - n4_d;
+    double hi = l4_d;
+    double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
+    double lop = lo > 0.0 ? lo : 0.0;
+    c.t1_lo_w = lop - tau;
+    c.t1_lo_n = lop + tau;
+    c.t1_hi_w = hi + tau;
+    c.t1_hi_n = hi - tau;
+    c.t2_w = -lo + tau;
+    c.t2_n = -lo - tau;
+    c.tau = tau;
+    c.lm = l4_d + m4;
+    bool need9 = l3 == 0 || m3 == 0;
+    bool need5 = l5 != 0 && m5 != 0;
+    uint32_t h = (need9 ? 9u : 3u) * (need5 ? 5u : 1u);
+    c.inv_h = inv_mod_2_24(h);
+    c.thr_h = 0xFFFFFFu / h;
+}
+
+// area4 = 2A: area4 = 0 (mod 8) and |area4| = 0 (mod h), from any va = A (mod 180)
+// with 0 <= va < 2^24
+HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) {
+    return (va & 3u) == 0 && ((va * c.inv_h) & 0xFFFFFFu) <= c.thr_h;
+}
+
+// Range terms of one orientation with d = 2D: term1 counts d = +|2D|, term2
+// d = -|2D|.  undecided is set when the FP64 estimate is within tau of a bound.
+HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, bool& undecided) {
+    double x = 2.0 * (Dd < 0 ? -Dd : Dd);
+    bool t1w = x >= c.t1_lo_w && x <= c.t1_hi_w;
+    bool t1n = x >= c.t1_lo_n && x <= c.t1_hi_n;
+    bool t2w = x <= c.t2_w;
+    bool t2n = x <= c.t2_n;
+    undecided = (t1w != t1n) || (t2w != t2n);
+    t1 = t1n;
+    t2 = t2n;
+}
+
+// zz estimates of the two terms
+HSS_HD inline double zz_term1(double Dd, const LaneCtx& c) { return c.lm - 2.0 * (Dd < 0 ? -Dd : Dd); }
+HSS_HD inline double zz_term2(double Dd, const LaneCtx& c) { return c.lm + 2.0 * (Dd < 0 ? -Dd : Dd); }
+
+// can an FP64 estimate zz (relative error < 2^-49, zz >= 1) be an exact fourth power?
+HSS_HD inline bool root_near_integer(double zz) {
+    double r = __builtin_sqrt(__builtin_sqrt(zz));
+    double f = r - __builtin_rint(r);
+    return (f < 0 ? -f : f) <= ROOT_EPS;
+}
+
+// One leaf z/2 = X + iY with area residues vX = X, vY = Y (mod 180).  cnt is
+// the decided candidate count; und is set when the leaf must be counted by the
+// exact stage instead; f[t] marks counted terms (orientation 1: term1, term2;
+// orientation 2: term1, term2) and zz[t] their FP64 zz estimates.
+struct LeafEval {
+    uint32_t cnt;
+    bool und;
+    bool f[4];
+    double zz[4];
+};
+
+HSS_HD inline void eval_leaf(double X, double Y, uint32_t vX, uint32_t vY, const LaneCtx& c, LeafEval& r) {
+    bool a1 = area_ok(vY, c);
+    bool a2 = area_ok(vX, c);
+    double ax = X < 0 ? -X : X, ay = Y < 0 ? -Y : Y;
+    double dm = X - Y, dp = X + Y;
+    bool eq = (dm < 0 ? -dm : dm) <= c.tau || (dp < 0 ? -dp : dp) <= c.tau;
+    bool und = (a2 && eq) || (a1 && ay <= c.tau) || (a2 && ax <= c.tau);
+    bool t1, t2, u1, t3, t4, u2;
+    range_terms(X, c, t1, t2, u1);
+    range_terms(Y, c, t3, t4, u2);
+    bool o2 = a2 && !eq;
+    und = und || (a1 && u1) || (o2 && u2);
+    r.f[0] = a1 && t1;
+    r.f[1] = a1 && t2;
+    r.f[2] = o2 && t3;
+    r.f[3] = o2 && t4;
+    r.cnt = (uint32_t)r.f[0] + (uint32_t)r.f[1] + (uint32_t)r.f[2] + (uint32_t)r.f[3];
+    r.und = und;
+    r.zz[0] = c.lm - 2.0 * ax;
+    r.zz[1] = c.lm + 2.0 * ax;
+    r.zz[2] = c.lm - 2.0 * ay;
+    r.zz[3] = c.lm + 2.0 * ay;
+}
+
+// The leaf u*v with both factors real (the central entries): z = 2uv is real,
+// so orientation 1 has area4 = 0 (never counted) and orientation 2 has d = 0,
+// counted once when area4 = 2uv passes the area tests and lo <= 0 (only the
+// positive sign: d = 0).  zz = l^4 + m^4 is never a fourth power (Fermat, n = 4),
+// so no term needs the root test.  vX = uv (mod 180).
+HSS_HD inline void eval_real_leaf(uint32_t vX, const LaneCtx& c, LeafEval& r) {
+    bool a2 = area_ok(vX, c);
+    // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
+    r.cnt = (a2 && c.t2_n >= 0.0) ? 1u : 0u;
+    r.und = a2 && c.t2_w >= 0.0 && c.t2_n < 0.0;
+    r.f[0] = r.f[1] = r.f[2] = r.f[3] = false;
+    r.zz[0] = r.zz[1] = r.zz[2] = r.zz[3] = 0.0;
+}
+
+// ------------------------------------------------------------ exact stage
+
+struct ExactCtx {
+    uint64_t l, m, limit;
+    u128 l4, m4, n4;
+    i128 lo, hi;
+    bool need9, need5;
+};
+
+HSS_HD inline void make_exact_ctx(uint64_t l, uint64_t m, uint64_t limit, ExactCtx& e) {
+    e.l = l;
+    e.m = m;
+    e.limit = limit;
+    e.l4 = (u128)l * l * l * l;
+    e.m4 = (u128)m * m * m * m;
+    e.n4 = (u128)limit * limit * limit * limit;
+    e.lo = (i128)e.l4 + (i128)e.m4 - (i128)e.n4;
+    e.hi = (i128)e.l4;
+    e.need9 = (l % 3 == 0) || (m % 3 == 0);
+    e.need5 = (l % 5 != 0) && (m % 5 != 0);
+}
+
+HSS_HD inline uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// 256-bit limb product a (na limbs) * b (nb limbs) -> out (8 limbs, truncated)
+HSS_HD inline void mul_limbs(const uint32_t* a, int na, const uint32_t* b, int nb, uint32_t* out) {
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < na; ++i) {
+        uint64_t carry = 0;
+        for (int j = 0; j < nb && i + j < 8; ++j) {
+            uint64_t v = (uint64_t)a[i] * b[j] + acc[i + j] + carry;
+            acc[i + j] = v & 0xFFFFFFFFu;
+            carry = v >> 32;
+        }
+        for (int j = i + nb; j < 8 && carry; ++j) {
+            uint64_t v = acc[j] + carry;
+            acc[j] = v & 0xFFFFFFFFu;
+            carry = v >> 32;
+        }
+    }
+    for (int i = 0; i < 8; ++i) out[i] = (uint32_t)acc[i];
+}
+
+// Heron product (x+y+z)(-x+y+z)(x-y+z)(x+y-z) == area4^2 for sides l^2, m^2,
+// n^2, all factors positive and 4 | area4 (src/search_v18.cpp:485-501)
+HSS_HD inline bool heron_matches(uint64_t l, uint64_t m, uint64_t n, u128 area4) {
+    i128 x = (i128)l * l, y = (i128)m * m, z = (i128)n * n;
+    i128 f[4] = {x + y + z, -x + y + z, x - y + z, x + y - z};
+    for (int i = 0; i < 4; ++i)
+        if (f[i] <= 0) return false;
+    if (area4 & 3) return false;
+    uint32_t p[8], b[4], t[8];
+    for (int i = 0; i < 8; ++i) p[i] = i < 4 ? (uint32_t)((u128)f[0] >> (32 * i)) : 0u;
+    for (int k = 1; k < 4; ++k) {
+        for (int i = 0; i < 4; ++i) b[i] = (uint32_t)((u128)f[k] >> (32 * i));
+        mul_limbs(p, 8, b, 4, t);
+        for (int i = 0; i < 8; ++i) p[i] = t[i];
+    }
+    uint32_t a[4];
+    for (int i = 0; i < 4; ++i) a[i] = (uint32_t)(area4 >> (32 * i));
+    mul_limbs(a, 4, a, 4, t);
+    for (int i = 0; i < 8; ++i)
+        if (p[i] != t[i]) return false;
+    return true;
+}
+
+// Fourth-power test of zz = l^4 + m^4 - d for one candidate: residues modulo
+// 16, 625, 9, 7, 13, 17 and the exact integer fourth root
+// (src/search_v18.cpp:457-475).  Returns n when zz = n^4, else 0.
+HSS_HD inline uint64_t fourth_root_of(const ExactCtx& e, i128 d, const QR4& q) {
+    i128 zs = (i128)e.l4 + (i128)e.m4 - d;
+    if (zs < 0) return 0;
+    u128 z = (u128)zs;
+    if (z < e.m4 || z > e.n4) return 0;
+    if ((uint32_t)(z & 15u) > 1u) return 0;
+    uint32_t r625 = umod128(z, 625);
+    if (e.need5 ? r625 != 0 : !bit_of(q.m625, r625)) return 0;
+    if (!((q.m9 >> umod128(z, 9)) & 1u) || !((q.m7 >> umod128(z, 7)) & 1u) ||
+        !((q.m13 >> umod128(z, 13)) & 1u) || !((q.m17 >> umod128(z, 17)) & 1u))
+        return 0;
+    double zd = (double)(uint64_t)(z >> 64) * 18446744073709551616.0 + (double)(uint64_t)z;
+    uint64_t r = (uint64_t)__builtin_sqrt(__builtin_sqrt(zd));
+    while (r > 0 && (u128)r * r * r * r > z) --r;
+    while ((u128)(r + 1) * (r + 1) * (r + 1) * (r + 1) <= z) ++r;
+    return (u128)r * r * r * r == z ? r : 0;
+}
+
+// Acceptance after an exact fourth power (src/search_v18.cpp:476-477 and the
+// recheck of :485-501): gcd(l, m, n) = 1, 5 | l*m*n, m <= n <= LIMIT, Heron.
+HSS_HD inline bool accept_hit(uint64_t l, uint64_t m, uint64_t n, uint64_t limit, u128 area4) {
+    if (gcd_u64(gcd_u64(l, m), n) != 1) return false;
+    if (l % 5 && m % 5 && n % 5) return false;
+    if (n < m || n > limit) return false;
+    return heron_matches(l, m, n, area4);
+}
+
+struct HitOut {
+    uint64_t l, m, n;
+    uint64_t area4_lo, area4_hi;  // |4A|
+};
+
+// Full reference filter chain of one leaf z (exact).  Returns the candidates;
+// every exact fourth power is passed to sink(n, |area4|).
+template <typename Sink>
+HSS_HD inline uint32_t exact_leaf(i128 zre, i128 zim, const ExactCtx& e, const QR4& q, Sink&& sink) {
+    uint32_t cand = 0;
+    for (int o = 0; o < 2; ++o) {
+        if (o == 1 && (zre == zim || zre == -zim)) break;
+        i128 d = o == 0 ? zre : zim;
+        i128 a4 = o == 0 ? zim : zre;
+        if (a4 == 0 || ((uint64_t)a4 & 7u)) continue;
+        bool pos = d >= e.lo && d <= e.hi;
+        bool neg = -d >= e.lo && -d <= e.hi;
+        if (!pos && !neg) continue;
+        u128 aa = (u128)(a4 < 0 ? -a4 : a4);
+        if (umod128(aa, 3)) continue;
+        if (e.need9 && umod128(aa, 9)) continue;
+        if (e.need5 && umod128(aa, 5)) continue;
+        for (int s = 0; s < 2; ++s) {
+            if (s == 0 ? !pos : (!neg || d == 0)) continue;
+            ++cand;
+            uint64_t r = fourth_root_of(e, s == 0 ? d : -d, q);
+            if (r) sink(r, aa);
+        }
+    }
+    return cand;
+}
+
+// exact Gaussian product z = 2 * u * v (conj: 2 * u * conj(v))
+HSS_HD inline void exact_product(i128 ure, i128 uim, int64_t vre, int64_t vim, bool conj, i128& zre, i128& zim) {
+    i128 c = vre, d = conj ? -(i128)vim : (i128)vim;
+    zre = 2 * (ure * c - uim * d);
+    zim = 2 * (ure * d + uim * c);
+}
+
+}  // namespace hss

@@ -1,0 +1,206 @@
+// hss_plan.cpp -- host-side batch planning: root factorisation, shared-prime
+// signatures, left-set recipes and tiles (see hss_plan.h).
+#include "hss_plan.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hss {
+
+std::vector<uint32_t> primes_upto(uint32_t upto) {
+    std::vector<uint8_t> comp(upto + 1, 0);
+    std::vector<uint32_t> out;
+    for (uint32_t i = 2; i <= upto; ++i) {
+        if (comp[i]) continue;
+        out.push_back(i);
+        for (uint64_t j = (uint64_t)i * i; j <= upto; j += i) comp[j] = 1;
+    }
+    return out;
+}
+
+uint32_t inv_mod_2_32(uint32_t p) {
+    uint32_t x = p;
+    for (int i = 0; i < 5; ++i) x *= 2u - p * x;
+    return x;
+}
+
+namespace {
+
+struct RootFactors {
+    uint64_t fixed = 1;
+    uint32_t ns = 0, ni = 0;
+    uint32_t sp[MAX_SPLIT], se[MAX_SPLIT];
+    uint32_t ip[MAX_INERT];
+};
+
+bool factor_root(uint64_t l, const std::vector<uint32_t>& primes, RootFactors& f) {
+    uint64_t x = l;
+    auto take = [&](uint64_t p, uint32_t a) -> bool {
+        if (p % 4 == 1) {
+            if (f.ns == MAX_SPLIT) return false;
+            f.sp[f.ns] = (uint32_t)p;
+            f.se[f.ns] = a;
+            ++f.ns;
+        } else {
+            for (uint32_t t = 0; t < 2 * a; ++t) f.fixed *= p;
+            if (p % 4 == 3) {
+                if (f.ni == MAX_INERT) return false;
+                f.ip[f.ni++] = (uint32_t)p;
+            }
+        }
+        return true;
+    };
+    for (uint32_t p : primes) {
+        if ((uint64_t)p * p > x) break;
+        if (x % p) continue;
+        uint32_t a = 0;
+        while (x % p == 0) {
+            x /= p;
+            ++a;
+        }
+        if (!take(p, a)) return false;
+    }
+    if (x > 1 && !take(x, 1)) return false;
+    return true;
+}
+
+// partners' half-set entries with index in [a, b]
+inline uint64_t entries(const uint32_t* off, uint32_t a, uint32_t b) { return (uint64_t)off[b + 1] - off[a]; }
+
+}  // namespace
+
+bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const std::vector<uint32_t>& small_primes,
+                const uint32_t* off, uint64_t n_max, uint64_t waves, Plan& P, std::string& err) {
+    P = Plan();
+    if (limit < 1 || limit > n_max) {
+        err = "limit outside [1, n_max]";
+        return false;
+    }
+    P.pos.resize(n_roots);
+    for (uint32_t s = 0; s < n_roots; ++s) {
+        const uint64_t l = roots[s];
+        if (l < 1 || l > limit) {
+            err = "root outside [1, limit]";
+            return false;
+        }
+        RootFactors f;
+        if (!factor_root(l, small_primes, f)) {
+            err = "root has too many distinct prime factors";
+            return false;
+        }
+        PosDesc& D = P.pos[s];
+        D = PosDesc();
+        D.l = l;
+        double l2 = (double)(l * l);
+        D.l2_d = l2;
+        D.l4_d = l2 * l2;
+        D.l3 = (uint32_t)(l % 3);
+        D.l5 = (uint32_t)(l % 5);
+        D.l_even = (l & 1) ? 0u : 1u;
+        D.nsplit = f.ns;
+        D.ninert = f.ni;
+        for (uint32_t j = 0; j < f.ns; ++j) {
+            D.split_inv[j] = inv_mod_2_32(f.sp[j]);
+            D.split_thr[j] = 0xFFFFFFFFu / f.sp[j];
+        }
+        for (uint32_t j = 0; j < f.ni; ++j) {
+            D.inert_inv[j] = inv_mod_2_32(f.ip[j]);
+            D.inert_thr[j] = 0xFFFFFFFFu / f.ip[j];
+        }
+        // signatures: exponent vectors s_j >= 0 with q = prod p_j^s_j <= limit
+        uint32_t sexp[MAX_SPLIT] = {0};
+        auto emit = [&](uint64_t q) {
+            uint64_t kmin = (l + q - 1) / q, kmax = limit / q;
+            if (kmin > kmax) return;
+            LeftBuild B{};
+            B.fixed = f.fixed;
+            B.k = f.ns;
+            uint64_t W = 1;
+            for (uint32_t j = 0; j < f.ns; ++j) {
+                B.p[j] = f.sp[j];
+                B.E[j] = f.se[j] + sexp[j];
+                W *= 4ull * B.E[j] + 1;
+            }
+            LeftDesc L{};
+            L.pos = s;
+            L.off = (uint32_t)P.left_entries;
+            L.H = (uint32_t)((W + 1) / 2);
+            L.q = (uint32_t)q;
+            L.trivial = f.ns == 0 ? 1u : 0u;
+            L.q3 = (uint32_t)(q % 3);
+            L.q5 = (uint32_t)(q % 5);
+            P.left_entries += L.H;
+            P.lefts.push_back(L);
+            P.builds.push_back(B);
+            P.kmin.push_back((uint32_t)kmin);
+            P.kmax.push_back((uint32_t)kmax);
+            P.est_cells += (double)L.H * (double)entries(off, (uint32_t)kmin, (uint32_t)kmax);
+        };
+        // iterative odometer over exponent vectors
+        uint64_t qpow[MAX_SPLIT + 1];
+        qpow[0] = 1;
+        std::fill(sexp, sexp + MAX_SPLIT, 0u);
+        for (uint32_t j = 0; j < f.ns; ++j) qpow[j + 1] = qpow[j];
+        for (;;) {
+            emit(qpow[f.ns]);
+            // increment: last position first
+            int j = (int)f.ns - 1;
+            while (j >= 0) {
+                uint64_t nq = qpow[j + 1] * f.sp[j];
+                if (nq <= limit) {
+                    ++sexp[j];
+                    qpow[j + 1] = nq;
+                    for (uint32_t t = (uint32_t)j + 1; t < f.ns; ++t) {
+                        sexp[t] = 0;
+                        qpow[t + 1] = qpow[t];
+                    }
+                    break;
+                }
+                --j;
+            }
+            if (j < 0) break;
+        }
+    }
+    if (P.left_entries > 0xFFFFFFFFull) {
+        err = "batch too large: split it";
+        return false;
+    }
+    // tile size: a few tiles per resident wavefront, bounded
+    double target = P.est_cells / (double)(waves ? waves * 8 : 8);
+    target = std::min(std::max(target, 8192.0), 4194304.0);
+    struct Cost {
+        double c;
+        uint32_t i;
+    };
+    std::vector<Cost> order;
+    for (uint32_t li = 0; li < P.lefts.size(); ++li) {
+        const LeftDesc& L = P.lefts[li];
+        uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
+        uint64_t budget = (uint64_t)std::max(target / (double)L.H, 256.0);
+        while (k0 <= kmax) {
+            // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
+            uint32_t lo = k0, hi = kmax;
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (entries(off, k0, mid) <= budget)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
+            TileDesc T{li, k0, kend, 0};
+            order.push_back({(double)L.H * (double)entries(off, k0, kend), (uint32_t)P.tiles.size()});
+            P.tiles.push_back(T);
+            if (kend == kmax) break;
+            k0 = kend + 1;
+        }
+    }
+    // heaviest tiles first
+    std::sort(order.begin(), order.end(), [](const Cost& a, const Cost& b) { return a.c > b.c; });
+    std::vector<TileDesc> sorted(P.tiles.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = P.tiles[order[i].i];
+    P.tiles.swap(sorted);
+    return true;
+}
+
+}  // namespace hss

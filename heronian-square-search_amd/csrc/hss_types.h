@@ -1,0 +1,71 @@
+// hss_types.h -- plain data shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "hss_math.h"
+
+namespace hss {
+
+static constexpr int MAX_SPLIT = 8;  // distinct primes = 1 (mod 4) of an integer <= 2^26
+static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
+
+// One root position l of a batch (uniform across a tile).
+struct PosDesc {
+    uint64_t l;
+    double l4_d;  // FP64 value of l^4
+    double l2_d;  // l^2 (exact)
+    uint32_t l3, l5;  // l mod 3, l mod 5
+    uint32_t l_even;
+    uint32_t nsplit, ninert;
+    uint32_t pad;
+    // divisibility of a partner index k by the split / inert primes of l:
+    // p | k  <=>  k * inv (mod 2^32) <= thr
+    uint32_t split_inv[MAX_SPLIT], split_thr[MAX_SPLIT];
+    uint32_t inert_inv[MAX_INERT], inert_thr[MAX_INERT];
+};
+
+// One left set: the half-set of Gaussian products for root l and one
+// signature of shared split primes (hss_runtime.cpp, plan_batch).
+struct LeftDesc {
+    uint32_t pos;      // position index in the batch
+    uint32_t off;      // first entry in the left arrays
+    uint32_t H;        // entries; entry H-1 is the real (central) one
+    uint32_t q;        // product of the shared prime powers: m = q * k
+    uint32_t trivial;  // l has no split prime: partners without one are skipped
+    uint32_t q3, q5;   // q mod 3, q mod 5
+    uint32_t pad;
+};
+
+// Build recipe of one left set: fixed part and merged exponents of l's split primes.
+struct LeftBuild {
+    uint64_t fixed;
+    uint32_t k;
+    uint32_t p[MAX_SPLIT];
+    uint32_t E[MAX_SPLIT];
+};
+
+// One unit of scheduled work: a left set against partner indices k in [k0, k1].
+struct TileDesc {
+    uint32_t left;
+    uint32_t k0, k1;
+    uint32_t pad;
+};
+
+// FP64 view of a left entry used by the hot loop (one scalar load).
+struct LeftFast {
+    double re, im;  // FP64 values (exact below 2^53, else correctly rounded)
+    uint32_t r180;  // (re mod 180) | (im mod 180) << 16
+    uint32_t pad;
+};
+
+// A leaf sent to the exact kernel.  kflags = k | conj << 30 | full << 31, where
+// full = 1 means the fast path left the leaf's count undecided.
+struct DefItem {
+    uint32_t left;
+    uint32_t ui;    // left entry index within the left set
+    uint32_t vidx;  // global half-set entry of the partner
+    uint32_t kflags;
+};
+
+}  // namespace hss

@@ -1,0 +1,121 @@
+/*
+ * hss.h -- C ABI of the MI355X (gfx950) Gaussian-integer enumeration core for
+ * the square-sided primitive Heronian triangle search.
+ *
+ * The reference has no in-process FFI: its replaceable unit is the
+ * gpu_search_v42 executable, whose host loop (src/gpu_search_v42.cu:393-506)
+ * prepares number-theory tables once, then launches search_kernel on batches of
+ * at most 512 root positions and reads back a candidate counter and hits.
+ * These entry points are that boundary, made callable: plain pointers and
+ * integers, no C++ or torch types.  The executables gpu_search / search in
+ * heronian-square-search_amd/ are thin C++ drivers over this ABI; a ctypes
+ * binding is heronian-square-search_amd/hss.py.
+ *
+ * Ownership: a context owns all device memory; callers own every host buffer
+ * they pass.  Errors: every function returns an int status (HSS_OK = 0) and
+ * never aborts; hss_last_error() gives a thread-local message.  Threading: a
+ * context is driven by one host thread at a time.
+ */
+#ifndef HSS_H
+#define HSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSS_OK 0
+#define HSS_EINVAL 1    /* bad argument (range, limit above the context's N_max, ...) */
+#define HSS_ENOMEM 2    /* device or host allocation failed */
+#define HSS_EDEVICE 3   /* HIP runtime error or kernel failure */
+#define HSS_EOVERFLOW 4 /* more hits than the caller's buffer holds (count still reported) */
+
+typedef struct hss_ctx hss_ctx;
+
+/* One accepted primitive triple: roots l <= m <= n, |4A| as two 64-bit halves. */
+typedef struct {
+    uint64_t l, m, n;
+    uint64_t area4_lo, area4_hi;
+} hss_hit;
+
+/* Per-root result of a search call. */
+typedef struct {
+    uint64_t candidates; /* entries of the fourth-power test (reference counter) */
+    uint64_t leaves;     /* Gaussian representations enumerated */
+    uint64_t pairs;      /* (l, m) pairs that reached the enumeration */
+} hss_root_result;
+
+/* Timings (milliseconds, HIP events on the context's stream) and work of the
+ * most recent search call. */
+typedef struct {
+    double plan_ms;    /* host: factor roots, signatures, tiles */
+    double setup_ms;   /* uploads + left-set build kernel */
+    double cells_ms;   /* cell_kernel (the enumeration) */
+    double exact_ms;   /* exact kernel + acceptance kernel */
+    double total_ms;   /* whole call, wall clock */
+    uint64_t tiles;
+    uint64_t left_entries;
+    uint64_t deferred;  /* leaves settled by the exact kernel */
+    uint64_t raw_roots; /* exact fourth powers before the acceptance rules */
+    uint64_t leaves;
+    uint64_t candidates;
+    uint32_t grid_blocks;
+    uint32_t block_threads;
+} hss_stats;
+
+/*
+ * Create a context for every LIMIT <= n_max (n_max <= 2^26) on HIP device
+ * `device`: on-device smallest-prime-factor sieve, Gaussian splits of the
+ * primes p = 1 (mod 4), and the half-set cache of every integer <= n_max, all
+ * resident in HBM.  Replaces Search::prepare and the table upload of
+ * src/gpu_search_v42.cu:354-432 (and src/search_v18.cpp:159-197, 285-309).
+ */
+int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out);
+
+/* Release a context (src/gpu_search_v42.cu:496-505). */
+void hss_ctx_destroy(hss_ctx* ctx);
+
+/* Thread-local description of the last failure in this thread. */
+const char* hss_last_error(void);
+
+/* Largest LIMIT the context serves, and its device memory footprint in bytes. */
+uint64_t hss_ctx_nmax(const hss_ctx* ctx);
+uint64_t hss_ctx_device_bytes(const hss_ctx* ctx);
+
+/*
+ * Search root positions roots[0..n_roots) at LIMIT `limit` (<= n_max) in one
+ * persistent launch: every pair (l, m), l <= m <= limit, is enumerated exactly
+ * as the reference's per-pair loop (src/search_v18.cpp:329-479) and the
+ * reference's candidate counter is reproduced bit for bit.  Roots may repeat
+ * and come in any order.  per_root (nullable, n_roots entries) receives each
+ * root's counters; *candidates_total (nullable) their sum.  Accepted hits
+ * (de-duplicated on (l, m, n), sorted) go to hits[0..hit_cap) and their number
+ * to *n_hits; HSS_EOVERFLOW when it exceeds hit_cap.  Replaces one
+ * search_kernel launch + synchronize + read-back of src/gpu_search_v42.cu:457-481,
+ * generalised from a contiguous batch to any list of roots.
+ */
+int hss_search(hss_ctx* ctx, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
+               hss_root_result* per_root, uint64_t* candidates_total, hss_hit* hits,
+               uint32_t hit_cap, uint32_t* n_hits);
+
+/* hss_search over the contiguous roots [first, last] (one reference batch). */
+int hss_search_interval(hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t last,
+                        uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap,
+                        uint32_t* n_hits);
+
+/* Timings and work counters of the last hss_search / hss_search_interval call. */
+int hss_get_stats(const hss_ctx* ctx, hss_stats* out);
+
+/*
+ * Host-side exact check of a result line "l m n area" (area = |4A|/4): the
+ * counterpart of scripts/verify.py:6-22 and of the 256-bit recheck in
+ * src/search_v18.cpp:481-507.  Returns 1 when (l^2, m^2, n^2) is a
+ * nondegenerate primitive triangle of that integer area, else 0.
+ */
+int hss_verify_triple(uint64_t l, uint64_t m, uint64_t n, uint64_t area_lo, uint64_t area_hi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

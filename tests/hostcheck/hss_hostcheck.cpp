@@ -1,0 +1,266 @@
+// hss_hostcheck.cpp -- CPU execution of the kernels' arithmetic, for tests only.
+//
+// TEST INFRASTRUCTURE.  The product (libhss.so, gpu_search, search) never links
+// or loads this library.  It runs, sequentially on the host, exactly the
+// per-entry and per-cell functions that the gfx950 kernels run (hss_build.h,
+// hss_math.h) over the same plan (hss_plan.cpp): the half-set cache build, the
+// left-set build, the cell evaluation with its FP64 range band and root
+// proximity filter, and the exact stage.  Its counts must equal the oracle's;
+// that checks the algorithm and the error bounds on the CPU, so that a GPU run
+// only has to confirm the parallel mapping.
+#include <stdint.h>
+
+#include <algorithm>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "hss_build.h"
+#include "hss_plan.h"
+
+using namespace hss;
+
+namespace {
+
+struct HostTables {
+    uint64_t n = 0;
+    std::vector<uint32_t> spf, off;
+    std::vector<double> cre, cim;
+    std::vector<uint32_t> cres;
+    std::vector<int64_t> pa, pb;  // per integer p (split primes only)
+    std::vector<uint32_t> small;
+};
+
+void factor(const HostTables& T, uint32_t i, int64_t& fixed, uint32_t* ps, uint32_t* es, uint32_t& k) {
+    fixed = 1;
+    k = 0;
+    uint32_t x = i;
+    while (x > 1) {
+        uint32_t p = T.spf[x], a = 0;
+        while (x % p == 0) {
+            x /= p;
+            ++a;
+        }
+        if ((p & 3) == 1) {
+            ps[k] = p;
+            es[k] = a;
+            ++k;
+        } else {
+            for (uint32_t j = 0; j < 2 * a; ++j) fixed *= p;
+        }
+    }
+}
+
+void build(HostTables& T, uint64_t n) {
+    T.n = n;
+    T.spf.assign(n + 1, 0);
+    for (uint64_t i = 0; i <= n; ++i) T.spf[i] = (uint32_t)i;
+    for (uint64_t p = 2; p * p <= n; ++p)
+        if (T.spf[p] == p)
+            for (uint64_t j = p * p; j <= n; j += p) T.spf[j] = std::min<uint32_t>(T.spf[j], (uint32_t)p);
+    T.pa.assign(n + 1, 0);
+    T.pb.assign(n + 1, 0);
+    for (uint64_t p = 5; p <= n; p += 4)
+        if (T.spf[p] == p) prime_split(p, T.pa[p], T.pb[p]);
+    T.off.assign(n + 2, 0);
+    for (uint64_t i = 1; i <= n; ++i) {
+        int64_t fixed;
+        uint32_t ps[MAX_SPLIT], es[MAX_SPLIT], k;
+        factor(T, (uint32_t)i, fixed, ps, es, k);
+        uint32_t w = 1;
+        for (uint32_t j = 0; j < k; ++j) w *= 4 * es[j] + 1;
+        T.off[i + 1] = T.off[i] + (w + 1) / 2;
+    }
+    uint32_t total = T.off[n + 1];
+    T.cre.resize(total);
+    T.cim.resize(total);
+    T.cres.resize(total);
+    for (uint64_t i = 1; i <= n; ++i) {
+        int64_t fixed;
+        uint32_t ps[MAX_SPLIT], es[MAX_SPLIT], k;
+        factor(T, (uint32_t)i, fixed, ps, es, k);
+        for (uint32_t e = T.off[i]; e < T.off[i + 1]; ++e) {
+            uint32_t dig[MAX_SPLIT];
+            decode_rank(e - T.off[i], k, es, dig);
+            int64_t re = fixed, im = 0;
+            for (uint32_t j = 0; j < k; ++j) {
+                GInt v = gauss_value(T.pa[ps[j]], T.pb[ps[j]], es[j], dig[j]);
+                int64_t a = (int64_t)v.re, b = (int64_t)v.im;
+                int64_t nr = re * a - im * b, ni = re * b + im * a;
+                re = nr;
+                im = ni;
+            }
+            T.cre[e] = (double)re;
+            T.cim[e] = (double)im;
+            int64_t r1 = re % (int64_t)QA, i1 = im % (int64_t)QA;
+            T.cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
+        }
+    }
+    uint32_t root = 2;
+    while ((uint64_t)root * root <= n) ++root;
+    T.small = primes_upto(root);
+}
+
+HostTables g_tables;
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+    uint64_t l, m, n, area4_lo, area4_hi;
+} hc_hit;
+
+typedef struct {
+    uint64_t deferred_full;  // leaves left undecided by the FP64 band
+    uint64_t deferred_root;  // counted terms near an exact fourth root
+    uint64_t raw_roots;
+    uint64_t cells;
+} hc_stats;
+
+// Search roots at limit with the kernels' arithmetic.  Tables are rebuilt when
+// nmax changes.  Returns 0 on success.
+int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint64_t* cand,
+              uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits, hc_stats* st) {
+    if (g_tables.n != nmax) build(g_tables, nmax);
+    const HostTables& T = g_tables;
+    Plan P;
+    std::string err;
+    if (!plan_batch(limit, roots, n_roots, T.small, T.off.data(), nmax, 4096, P, err)) return 1;
+    // left sets
+    std::vector<i128> lre(P.left_entries), lim(P.left_entries);
+    std::vector<LeftFast> lfast(P.left_entries);
+    for (size_t li = 0; li < P.lefts.size(); ++li) {
+        const LeftBuild& b = P.builds[li];
+        const LeftDesc& L = P.lefts[li];
+        for (uint32_t t = 0; t < L.H; ++t) {
+            uint32_t dig[MAX_SPLIT];
+            decode_rank(t, b.k, b.E, dig);
+            GInt z = {(i128)b.fixed, 0};
+            for (uint32_t j = 0; j < b.k; ++j) z = gmul(z, gauss_value(T.pa[b.p[j]], T.pb[b.p[j]], b.E[j], dig[j]));
+            lre[L.off + t] = z.re;
+            lim[L.off + t] = z.im;
+            LeftFast f;
+            f.re = i128_to_double(z.re);
+            f.im = i128_to_double(z.im);
+            f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
+            f.pad = 0;
+            lfast[L.off + t] = f;
+        }
+    }
+    std::fill(cand, cand + n_roots, 0);
+    std::fill(leaves, leaves + n_roots, 0);
+    std::fill(pairs, pairs + n_roots, 0);
+    hc_stats S{};
+    struct Def {
+        uint32_t left, ui, vidx, k, conj, full;
+    };
+    std::vector<Def> defs;
+    double ld = (double)(limit * limit), n4 = ld * ld;
+    for (const TileDesc& Tl : P.tiles) {
+        const LeftDesc& L = P.lefts[Tl.left];
+        const PosDesc& D = P.pos[L.pos];
+        for (uint32_t k = Tl.k0; k <= Tl.k1; ++k) {
+            bool ok = !(D.l_even && !(k & 1u));
+            for (uint32_t j = 0; j < D.nsplit; ++j) ok = ok && !(k * D.split_inv[j] <= D.split_thr[j]);
+            for (uint32_t j = 0; j < D.ninert; ++j) ok = ok && !(k * D.inert_inv[j] <= D.inert_thr[j]);
+            if (!ok) continue;
+            uint32_t Hk = T.off[k + 1] - T.off[k];
+            if (L.trivial && Hk == 1) continue;
+            pairs[L.pos] += 1;
+            uint64_t m = (uint64_t)L.q * k;
+            LaneCtx ctx;
+            make_lane_ctx(m, D.l4_d, D.l2_d, n4, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
+            for (uint32_t jj = 0; jj < Hk; ++jj) {
+                uint32_t vidx = T.off[k] + jj;
+                bool vc = jj + 1 == Hk;
+                double c = T.cre[vidx], d = T.cim[vidx];
+                uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
+                leaves[L.pos] += vc ? L.H : 2ull * L.H - 1;
+                for (uint32_t ui = 0; ui < L.H; ++ui) {
+                    const LeftFast& u = lfast[L.off + ui];
+                    bool uc = ui + 1 == L.H;
+                    uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                    double t1 = u.im * d, t2 = u.re * d;
+                    uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
+                    ++S.cells;
+                    for (uint32_t cj = 0; cj < 2; ++cj) {
+                        if (cj == 1 && (uc || vc)) break;
+                        double X, Y;
+                        uint32_t vX, vY;
+                        if (cj == 0) {
+                            X = __builtin_fma(u.re, c, -t1);
+                            Y = __builtin_fma(u.im, c, t2);
+                            vX = pac + QA * QA - pbd;
+                            vY = pad + pbc;
+                        } else {
+                            X = __builtin_fma(u.re, c, t1);
+                            Y = __builtin_fma(u.im, c, -t2);
+                            vX = pac + pbd;
+                            vY = pbc + QA * QA - pad;
+                        }
+
+                        LeafEval r;
+                        if (cj == 0 && uc && vc)
+                            eval_real_leaf(vX, ctx, r);  // both factors real
+                        else
+                            eval_leaf(X, Y, vX, vY, ctx, r);
+                        if (r.und) {
+                            defs.push_back({Tl.left, ui, vidx, k, cj, 1});
+                            ++S.deferred_full;
+                            continue;
+                        }
+                        cand[L.pos] += r.cnt;
+                        for (int tm = 0; tm < 4; ++tm)
+                            if (r.f[tm] && root_near_integer(r.zz[tm])) {
+                                defs.push_back({Tl.left, ui, vidx, k, cj, 0});
+                                ++S.deferred_root;
+                            }
+                    }
+                }
+            }
+        }
+    }
+    // exact stage + acceptance
+    QR4 qr = make_qr4();
+    std::set<std::tuple<uint64_t, uint64_t, uint64_t>> seen;
+    std::vector<hc_hit> got;
+    for (const Def& df : defs) {
+        const LeftDesc& L = P.lefts[df.left];
+        const PosDesc& D = P.pos[L.pos];
+        uint64_t m = (uint64_t)L.q * df.k;
+        ExactCtx e;
+        make_exact_ctx(D.l, m, limit, e);
+        i128 zre, zim;
+        exact_product(lre[L.off + df.ui], lim[L.off + df.ui], (int64_t)T.cre[df.vidx], (int64_t)T.cim[df.vidx],
+                      df.conj != 0, zre, zim);
+        uint32_t c = exact_leaf(zre, zim, e, qr, [&](uint64_t r, u128 a4) {
+            ++S.raw_roots;
+            if (accept_hit(D.l, m, r, limit, a4) && seen.insert({D.l, m, r}).second)
+                got.push_back(hc_hit{D.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64)});
+        });
+        if (df.full) cand[L.pos] += c;
+    }
+    std::sort(got.begin(), got.end(),
+              [](const hc_hit& a, const hc_hit& b) { return std::tie(a.l, a.m, a.n) < std::tie(b.l, b.m, b.n); });
+    for (uint32_t i = 0; i < got.size() && i < hit_cap; ++i) hits[i] = got[i];
+    *n_hits = (uint32_t)got.size();
+    if (st) *st = S;
+    return 0;
+}
+
+// Cache entry e of the host tables (for cross-checks of the table layout).
+int hc_cache_entry(uint64_t nmax, uint32_t i, uint32_t t, double* re, double* im, uint32_t* r180, uint32_t* H) {
+    if (g_tables.n != nmax) build(g_tables, nmax);
+    const HostTables& T = g_tables;
+    if (i < 1 || i > nmax) return 1;
+    *H = T.off[i + 1] - T.off[i];
+    if (t >= *H) return 1;
+    *re = T.cre[T.off[i] + t];
+    *im = T.cim[T.off[i] + t];
+    *r180 = T.cres[T.off[i] + t];
+    return 0;
+}
+
+}  // extern "C"
