@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: verified root positions per second at LIMIT = 10^7 on one MI355X.
+
+Workload (config.workload) "uneven1536": one batch of 1536 root positions at
+LIMIT 10^7 -- the reference's interval l in [7500000, 7500511]
+(evidence/benchmarks/gpu-opt-benchmark.json:15-20), 512 heavy roots (at least
+four distinct primes = 1 mod 4, Gaussian weight >= 625) and 512 light prime-power
+roots, both drawn from [9e6, 1e7] with seed 20261203.  A step is one search of
+the whole batch: one persistent launch of the cell kernel plus the left-set
+build, exact and acceptance kernels, with the number-theory tables already
+resident in HBM (built once, outside the timed region).  Every step's total
+candidate count is checked against the first step's, and the slice's total
+against the reference's 39,091,522,776 (bit-exact or the run fails).
+
+Prints one JSON line.  roofline: the cell kernel is bound by the vector ALU
+(integer and FP64 lane operations), not by HBM or MFMA; achieved uses the
+algorithmic work 64 * leaves + 32 * candidates lane-ops per launch divided by
+the kernel's HIP-event duration; peak = 256 CU x 64 lanes x 2.4 GHz.
+cpu_baseline: the reference search_v18 compiled from source (oracle/_ref/search),
+timed on this host on a bounded sample of the same workload and scaled by the
+exact leaf counts to the whole batch.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "heronian-square-search_amd"
+sys.path.insert(0, str(PKG))
+
+METRIC = "verified root-side positions/sec at LIMIT=1e7 (bit-exact); p50/p99 req latency"
+LIMIT = 10_000_000
+SLICE_CANDIDATES = 39_091_522_776  # evidence/benchmarks/gpu-opt-benchmark.json:15-20
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # lane-ops per second, in T
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(per_root_leaves: dict[int, int], total_leaves: int, positions: int) -> dict | None:
+    """Time the reference CPU search on a bounded sample of the workload."""
+    ref = ROOT / "oracle" / "_ref" / "search"
+    port = ROOT / "oracle" / "hss_oracle_cli"
+    first, last = 7_500_000, 7_500_007
+    threads = min(8, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as work:
+        env = dict(os.environ, HERON_COVERAGE_FILE=str(Path(work) / "none.json"))
+        if ref.is_file():
+            kind = "reference"
+            cmd = [str(ref), str(LIMIT), str(first), str(last), str(threads), str(Path(work) / "s"),
+                   str(Path(work) / "r")]
+        elif port.is_file():
+            kind = "port"
+            cmd = [str(port), str(LIMIT), str(first), str(last), str(threads)]
+        else:
+            return None
+        t0 = time.perf_counter()
+        subprocess.run(cmd, cwd=work, env=env, check=True, capture_output=True)
+        wall = time.perf_counter() - t0
+    sample_leaves = sum(per_root_leaves[l] for l in range(first, last + 1))
+    leaves_per_s = sample_leaves / wall
+    value = positions * leaves_per_s / total_leaves
+    return {"value": value, "unit": "positions/s", "cores": threads, "kind": kind,
+            "sample": (f"l in [{first},{last}] (8 positions of the batch) at LIMIT 1e7, {threads} threads, "
+                       f"{wall:.1f} s wall incl. table prepare; {leaves_per_s:.3e} leaves/s scaled to the "
+                       f"batch's {total_leaves} leaves")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="uneven1536", choices=["uneven1536", "slice512"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(root positions shard with no data exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import hss
+    import workloads
+
+    limit, roots = workloads.workload(args.workload)
+    assert limit == LIMIT
+    t0 = time.perf_counter()
+    ctx = hss.Context(LIMIT, hss.visible_device())
+    log(f"tables for N_max={LIMIT} built in {time.perf_counter() - t0:.2f} s "
+        f"({ctx.device_bytes / 2**30:.2f} GiB resident)")
+
+    def step():
+        r = ctx.search(LIMIT, roots)
+        return r
+
+    ref_total = None
+    for i in range(args.warmup):
+        r = step()
+        ref_total = r.total
+        log(f"warmup {i}: {r.stats['total_ms']:.1f} ms, candidates {r.total}, cells {r.stats['cells_ms']:.1f} ms")
+    per_root_leaves = dict(zip(roots, r.leaves)) if args.warmup else None
+    slice_total = sum(c for l, c in zip(roots, r.candidates) if 7_500_000 <= l <= 7_500_511) if args.warmup else None
+    if slice_total is not None and slice_total != SLICE_CANDIDATES:
+        log(f"PARITY FAILURE: slice candidates {slice_total} != {SLICE_CANDIDATES}")
+        return 1
+    cells_ms, exact_ms, stats = [], [], None
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        r = step()
+        if ref_total is None:
+            ref_total = r.total
+        if r.total != ref_total:
+            log(f"PARITY FAILURE: step {i} candidates {r.total} != {ref_total}")
+            return 1
+        cells_ms.append(r.stats["cells_ms"])
+        exact_ms.append(r.stats["exact_ms"])
+        stats = r.stats
+        log(f"step {i}: {r.stats['total_ms']:.1f} ms (cells {r.stats['cells_ms']:.1f} ms)")
+    elapsed = time.perf_counter() - t_start
+    if per_root_leaves is None:
+        per_root_leaves = dict(zip(roots, r.leaves))
+    positions = len(roots) * args.steps
+    value = positions / elapsed
+    leaves, cands = stats["leaves"], stats["candidates"]
+    cell_s = sum(cells_ms) / len(cells_ms) / 1e3
+    ops = 64 * leaves + 32 * cands
+    achieved = ops / cell_s / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "positions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+i32 (exact i128 settle)",
+        "data": "synthetic (seeded root positions; no dataset)",
+        "config": {"workload": f"{args.workload}: {len(roots)} root positions at LIMIT=1e7 "
+                               "(slice 7500000-7500511 + 512 heavy + 512 light roots, seed 20261203)",
+                   "limit": LIMIT, "positions": len(roots), "parallelism": "1 GPU, persistent grid"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "kernel": "cell_kernel", "ops_per_launch": ops, "kernel_ms": cell_s * 1e3},
+        "work": {"leaves": leaves, "candidates": cands, "tiles": stats["tiles"], "deferred": stats["deferred"],
+                 "exact_ms": sum(exact_ms) / len(exact_ms), "leaves_per_s": leaves / cell_s},
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline sample")
+        line["cpu_baseline"] = cpu_baseline(per_root_leaves, sum(per_root_leaves[l] for l in roots), len(roots))
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -274,11 +274,25 @@ __device__ inline bool divides(uint32_t k, uint32_t inv, uint32_t thr) { return 
 __device__ inline void push_global(const CellParams& P, bool pred, const DefItem& it) {
     uint64_t mask = __ballot(pred);
     if (!mask) return;
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(P.ndef, (uint32_t)__popcll(mask));
-    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+    // Every lane executes the atomic (lane 0 adds the count, the others 0) and the
+    // base is read with readfirstlane: no divergent branch may precede a
+    // cross-lane read inside the kernel's loops, or the compiler is free to let
+    // the lanes drift into different loop iterations.
+    uint32_t got = atomicAdd(P.ndef, __lane_id() == 0 ? (uint32_t)__popcll(mask) : 0u);
+    uint32_t base = __builtin_amdgcn_readfirstlane(got);
     uint32_t slot = base + lane_rank(mask);
     if (pred && slot < P.def_cap) P.def[slot] = it;
+}
+
+__device__ inline LeafEval select_leaf(bool c, const LeafEval& a, const LeafEval& b) {
+    LeafEval r;
+    r.cnt = c ? a.cnt : b.cnt;
+    r.und = c ? a.und : b.und;
+    for (int i = 0; i < 4; ++i) {
+        r.f[i] = c ? a.f[i] : b.f[i];
+        r.zz[i] = c ? a.zz[i] : b.zz[i];
+    }
+    return r;
 }
 
 // Per-wave LDS queue of counted terms awaiting the root-proximity test.
@@ -290,12 +304,10 @@ struct RootQueue {
 
 __device__ inline void root_drain(const CellParams& P, RootQueue& q, uint32_t left, uint32_t count) {
     const uint32_t lane = __lane_id();
-    bool pass = false;
-    uint4 id = make_uint4(0, 0, 0, 0);
-    if (lane < count) {
-        id = q.id[lane];
-        pass = root_near_integer(q.z[lane]);
-    }
+    const bool in = lane < count;
+    const uint32_t at = in ? lane : 0u;  // branch-free: every lane reads a valid slot
+    const uint4 id = q.id[at];
+    const bool pass = in && root_near_integer(q.z[at]);
     push_global(P, pass, DefItem{left, id.x, id.y, id.z});
 }
 
@@ -303,11 +315,11 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, uint32_t lef
                                  uint32_t ui, uint32_t vidx, uint32_t kc) {
     uint64_t mask = __ballot(pred);
     if (!mask) return;
-    uint32_t slot = q.n + lane_rank(mask);
-    if (pred) {
-        q.z[slot] = zz;
-        q.id[slot] = make_uint4(ui, vidx, kc, 0);
-    }
+    // lanes without an item write the dummy slot QCAP-1 (never a live entry:
+    // the queue holds at most 63 + 64 = 127 items), so the store is unconditional
+    const uint32_t slot = pred ? q.n + lane_rank(mask) : (uint32_t)(QCAP - 1);
+    q.z[slot] = zz;
+    q.id[slot] = make_uint4(ui, vidx, kc, 0);
     q.n += (uint32_t)__popcll(mask);
     if (q.n >= 64) {
         __builtin_amdgcn_wave_barrier();
@@ -317,10 +329,9 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, uint32_t lef
         double mz = q.z[64 + lane];
         uint4 mi = q.id[64 + lane];
         __builtin_amdgcn_wave_barrier();
-        if (lane + 64 < q.n) {
-            q.z[lane] = mz;
-            q.id[lane] = mi;
-        }
+        const uint32_t to = lane + 64 < q.n ? lane : (uint32_t)(QCAP - 1);
+        q.z[to] = mz;
+        q.id[to] = mi;
         __builtin_amdgcn_wave_barrier();
         q.n -= 64;
     }
@@ -332,9 +343,8 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     RootQueue rq{s_qz[wave], s_qid[wave], 0};
     for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(P.next_tile, 1u);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
+        const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_tile, lane == 0 ? 1u : 0u));
         if (t >= P.ntiles) break;
         const TileDesc T = P.tiles[t];
         const LeftDesc L = P.lefts[T.left];
@@ -343,17 +353,13 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
         for (uint32_t par = 0; par < 2; ++par) {
             uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
             for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
-                uint32_t k = kb + 2u * (uint32_t)lane;
-                uint32_t Hk = 0;
-                if (k <= T.k1) {
-                    bool ok = !(S.l_even && !(k & 1u));
-                    for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
-                    for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
-                    if (ok) {
-                        Hk = P.off[k + 1] - P.off[k];
-                        if (L.trivial && Hk == 1) Hk = 0;
-                    }
-                }
+                const uint32_t k = kb + 2u * (uint32_t)lane;
+                const uint32_t kq = k <= T.k1 ? k : T.k1;  // in-bounds index for every lane
+                bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
+                for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
+                for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
+                uint32_t Hk = P.off[kq + 1] - P.off[kq];
+                Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
                 pairs += Hk ? 1 : 0;
                 uint32_t incl = wave_incl_scan(Hk, lane);
                 uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
@@ -368,9 +374,10 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
                     j = j > 63 ? 63 : j;
                     uint32_t inclj = __shfl(incl, (int)j, 64);
                     uint32_t Hj = __shfl(Hk, (int)j, 64);
-                    uint32_t kj = kb + 2u * j;
-                    uint32_t jj = e - (inclj - Hj);
-                    uint32_t vidx = act ? P.off[kj] + jj : 0u;
+                    const uint32_t kj = kb + 2u * j;
+                    const uint32_t jj = e - (inclj - Hj);
+                    const uint32_t kjq = act ? kj : T.k0;  // in-bounds index for every lane
+                    const uint32_t vidx = act ? P.off[kjq] + jj : 0u;
                     bool vc = jj + 1 == Hj;
                     double c = P.cre[vidx], d = P.cim[vidx];
                     uint32_t vr = P.cres[vidx];
@@ -379,7 +386,7 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
                     uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
                     LaneCtx ctx;
                     make_lane_ctx(m, S.l4_d, S.l2_d, P.n4_d, S.l3, S.l5, m3, m5, ctx);
-                    if (act) leaves += vc ? L.H : 2ull * L.H - 1;
+                    leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
                     for (uint32_t ui = 0; ui < L.H; ++ui) {
                         const LeftFast u = P.lfast[L.off + ui];
                         const bool uc = ui + 1 == L.H;
@@ -403,12 +410,12 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
                                 vY = pbc + QA * QA - pad;
                             }
 
-                            LeafEval r;
-                            if (cj == 0 && uc && vc)
-                                eval_real_leaf(vX, ctx, r);  // both factors real
-                            else
-                                eval_leaf(X, Y, vX, vY, ctx, r);
-                            if (live && !r.und) cand += r.cnt;
+                            // both evaluations, then a select: no per-lane branch here
+                            LeafEval ra, rb;
+                            eval_leaf(X, Y, vX, vY, ctx, ra);
+                            eval_real_leaf(vX, ctx, rb);  // both factors real
+                            const LeafEval r = select_leaf(cj == 0 && uc && vc, rb, ra);
+                            cand += (live && !r.und) ? r.cnt : 0u;
                             const uint32_t kc = kj | cj << 30;
                             push_global(P, live && r.und, DefItem{T.left, ui, vidx, kc | 0x80000000u});
                             const bool ok = live && !r.und;
@@ -427,11 +434,10 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
         cand = wave_sum64(cand);
         leaves = wave_sum64(leaves);
         pairs = wave_sum64(pairs);
-        if (lane == 0) {
-            atomicAdd(P.cand + L.pos, (unsigned long long)cand);
-            atomicAdd(P.leaves + L.pos, (unsigned long long)leaves);
-            atomicAdd(P.pairs + L.pos, (unsigned long long)pairs);
-        }
+        // all lanes execute the atomics; only lane 0 contributes
+        atomicAdd(P.cand + L.pos, lane == 0 ? (unsigned long long)cand : 0ull);
+        atomicAdd(P.leaves + L.pos, lane == 0 ? (unsigned long long)leaves : 0ull);
+        atomicAdd(P.pairs + L.pos, lane == 0 ? (unsigned long long)pairs : 0ull);
     }
 }
 

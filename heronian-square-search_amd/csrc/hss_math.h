@@ -43,7 +43,12 @@ typedef __int128 i128;
 typedef unsigned __int128 u128;
 
 static constexpr uint32_t QA = 180;        // area residue modulus 4 * 9 * 5
-static constexpr double ROOT_EPS = 1e-4;   // root proximity band (errors < 1e-7)
+// Root proximity band.  For a counted term zz >= m^4 >= (l^4 + m^4)/2; the FP64
+// estimate carries |err| <= 2^-52 (l^4+m^4) [l^4 + m^4] + 2^-50 l^2 m^2 [2|D|,
+// left values possibly rounded] + 2^-53 zz [subtraction] < 2^-49.3 zz, so the
+// fourth root is off by < 2^-51.3 n plus two sqrt roundings: < 3e-8 for
+// n <= 2^26.  1e-6 keeps a 30x margin; random zz pass with probability 2e-6.
+static constexpr double ROOT_EPS = 1e-6;
 
 // ------------------------------------------------------------------ residues
 

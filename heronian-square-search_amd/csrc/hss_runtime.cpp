@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <set>
 #include <string>
@@ -60,6 +62,22 @@ struct DevBuf {
         return static_cast<T*>(p);
     }
 };
+
+// HSS_VERBOSE=1: synchronize and report after each phase (diagnostics only)
+bool verbose() {
+    static const bool v = std::getenv("HSS_VERBOSE") != nullptr;
+    return v;
+}
+
+void vlog(hipStream_t st, const char* what) {
+    if (!verbose()) return;
+    static const auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = st ? hipStreamSynchronize(st) : hipSuccess;
+    std::fprintf(stderr, "[hss] %-28s t=%8.3f s  %s\n", what,
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                 e == hipSuccess ? "" : hipGetErrorString(e));
+    std::fflush(stderr);
+}
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -152,8 +170,10 @@ static int build_tables(hss_ctx* c) {
         HIP_TRY(hipMemcpyAsync(d_small, sieve_primes.data(), sieve_primes.size() * 4, hipMemcpyHostToDevice, st));
     BuildBuffers B{n, d_spf, d_H, d_small, (uint32_t)sieve_primes.size()};
     HIP_TRY(launch_build_tables(B, st));
+    vlog(st, "sieve + half counts");
     HIP_TRY(hipMalloc(&c->d_off, (n + 2) * 4));
     HIP_TRY(launch_scan(d_H, c->d_off, d_sums, n + 2, st));
+    vlog(st, "offset scan");
     c->h_off.resize(n + 2);
     HIP_TRY(hipMemcpyAsync(c->h_off.data(), c->d_off, (n + 2) * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -172,6 +192,7 @@ static int build_tables(hss_ctx* c) {
     cleanup.ptrs.push_back(d_cnt);
     HIP_TRY(hipMemsetAsync(d_cnt, 0, 4, st));
     HIP_TRY(launch_collect_split(d_spf, n, d_plist, d_psize, d_cnt, st));
+    vlog(st, "split primes collected");
     uint32_t np = 0;
     HIP_TRY(hipMemcpyAsync(&np, d_cnt, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -193,12 +214,14 @@ static int build_tables(hss_ctx* c) {
         HIP_TRY(hipMalloc(&c->d_vre, (size_t)nvals * sizeof(i128)));
         HIP_TRY(hipMalloc(&c->d_vim, (size_t)nvals * sizeof(i128)));
         HIP_TRY(launch_prime_values(d_plist, np, d_vstart, nvals, c->d_vbase, d_pa, d_pb, c->d_vre, c->d_vim, st));
+        vlog(st, "prime values");
     }
     HIP_TRY(hipMalloc(&c->d_cre, std::max<size_t>(total, 1) * 8));
     HIP_TRY(hipMalloc(&c->d_cim, std::max<size_t>(total, 1) * 8));
     HIP_TRY(hipMalloc(&c->d_cres, std::max<size_t>(total, 1) * 4));
     HIP_TRY(launch_fill_cache(d_spf, c->d_off, n, (uint32_t)total, c->d_vbase, c->d_vre, c->d_vim, c->d_cre,
                               c->d_cim, c->d_cres, st));
+    vlog(st, "cache filled");
     HIP_TRY(hipStreamSynchronize(st));
     c->device_bytes = (n + 2) * 4 + (n + 1) * 4 + (uint64_t)nvals * 32 + total * 20;
     return HSS_OK;
@@ -221,7 +244,9 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
         g_error = keep;
         return code;
     };
+    vlog(nullptr, "device query");
     if (hipSetDevice(device) != hipSuccess) return bail(fail(HSS_EDEVICE, "hipSetDevice failed"));
+    vlog(nullptr, "device set");
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(HSS_EDEVICE, "device properties"));
     c->cus = prop.multiProcessorCount;
@@ -232,7 +257,9 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
     int bpc = 0;
     if (cell_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
     c->blocks_per_cu = bpc;
+    vlog(nullptr, "stream + occupancy");
     int rc = build_tables(c);
+    vlog(c->stream, "tables built");
     if (rc != HSS_OK) return bail(rc);
     *out = c;
     return HSS_OK;
@@ -316,7 +343,9 @@ static int run_batch(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t
         HIP_TRY(hipMemsetAsync(c->counters.p, 0, (size_t)npos * 3 * 8, st));
         HIP_TRY(hipMemsetAsync(misc, 0, 16, st));
         HIP_TRY(hipEventRecord(c->ev[1], st));
+        vlog(st, "left sets built");
         if (ntiles) HIP_TRY(launch_cells(K, blocks, st));
+        vlog(st, "cell kernel");
         HIP_TRY(hipEventRecord(c->ev[2], st));
         HIP_TRY(hipMemcpyAsync(&ndef, misc + 1, 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -327,6 +356,7 @@ static int run_batch(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t
         HIP_TRY(c->def.reserve((size_t)c->def_cap * sizeof(DefItem)));
     }
     HIP_TRY(launch_exact(K, ndef, st));
+    vlog(st, "exact kernel");
     HIP_TRY(launch_accept(K.raw, K.nraw, raw_cap, limit, c->hits.as<HitOut>(), misc + 3, hit_cap, st));
     HIP_TRY(hipEventRecord(c->ev[3], st));
     cnt.resize((size_t)npos * 3);

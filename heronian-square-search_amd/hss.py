@@ -1,0 +1,159 @@
+"""ctypes binding of the C ABI in include/hss.h (libhss.so beside this file).
+
+The library is the product's only compute path: there is no CPU fallback, and
+importing this module raises when libhss.so has not been built
+(``make -C heronian-square-search_amd``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libhss.so"
+
+HSS_OK, HSS_EINVAL, HSS_ENOMEM, HSS_EDEVICE, HSS_EOVERFLOW = 0, 1, 2, 3, 4
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_uint64), ("m", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("area4_lo", ctypes.c_uint64), ("area4_hi", ctypes.c_uint64)]
+
+    @property
+    def area(self) -> int:
+        return ((self.area4_hi << 64) | self.area4_lo) // 4
+
+    def line(self) -> str:
+        return f"{self.l} {self.m} {self.n} {self.area}"
+
+
+class RootResult(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_uint64), ("leaves", ctypes.c_uint64), ("pairs", ctypes.c_uint64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("plan_ms", ctypes.c_double), ("setup_ms", ctypes.c_double), ("cells_ms", ctypes.c_double),
+                ("exact_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("tiles", ctypes.c_uint64),
+                ("left_entries", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("raw_roots", ctypes.c_uint64),
+                ("leaves", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
+                ("block_threads", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
+    if not path.is_file():
+        raise RuntimeError(f"{path} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
+    lib = ctypes.CDLL(str(path))
+    u64, u32, vp = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p
+    lib.hss_ctx_create.argtypes = [u64, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.hss_ctx_destroy.argtypes = [vp]
+    lib.hss_ctx_destroy.restype = None
+    lib.hss_last_error.restype = ctypes.c_char_p
+    lib.hss_ctx_nmax.argtypes = [vp]
+    lib.hss_ctx_nmax.restype = u64
+    lib.hss_ctx_device_bytes.argtypes = [vp]
+    lib.hss_ctx_device_bytes.restype = u64
+    lib.hss_search.argtypes = [vp, u64, ctypes.POINTER(u64), u32, ctypes.POINTER(RootResult),
+                               ctypes.POINTER(u64), ctypes.POINTER(Hit), u32, ctypes.POINTER(u32)]
+    lib.hss_search_interval.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(Hit), u32,
+                                        ctypes.POINTER(u32)]
+    lib.hss_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.hss_verify_triple.argtypes = [u64, u64, u64, u64, u64]
+    return lib
+
+
+_lib = load_library()
+
+
+class HssError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hss error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != HSS_OK:
+        raise HssError(rc, _lib.hss_last_error().decode(errors="replace"))
+
+
+def verify_triple(l: int, m: int, n: int, area: int) -> bool:
+    """Exact Heron check of a result line (the counterpart of scripts/verify.py)."""
+    return bool(_lib.hss_verify_triple(l, m, n, area & ((1 << 64) - 1), area >> 64))
+
+
+@dataclass
+class SearchResult:
+    candidates: list[int]
+    leaves: list[int]
+    pairs: list[int]
+    hits: list[tuple[int, int, int, int]]
+    stats: dict
+
+    @property
+    def total(self) -> int:
+        return sum(self.candidates)
+
+
+class Context:
+    """Device-resident tables for every LIMIT <= n_max on one HIP device."""
+
+    def __init__(self, n_max: int, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(_lib.hss_ctx_create(n_max, device, ctypes.byref(self._h)))
+        self.n_max = n_max
+
+    def close(self):
+        if self._h:
+            _lib.hss_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        return _lib.hss_ctx_device_bytes(self._h)
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(_lib.hss_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def search(self, limit: int, roots, hit_cap: int = 4096) -> SearchResult:
+        roots = [int(r) for r in roots]
+        n = len(roots)
+        arr = (ctypes.c_uint64 * max(n, 1))(*roots)
+        per = (RootResult * max(n, 1))()
+        total = ctypes.c_uint64()
+        hits = (Hit * max(hit_cap, 1))()
+        nh = ctypes.c_uint32()
+        _check(_lib.hss_search(self._h, limit, arr, n, per, ctypes.byref(total), hits, hit_cap, ctypes.byref(nh)))
+        return SearchResult([per[i].candidates for i in range(n)], [per[i].leaves for i in range(n)],
+                            [per[i].pairs for i in range(n)],
+                            [(hits[i].l, hits[i].m, hits[i].n, hits[i].area) for i in range(nh.value)],
+                            self.stats())
+
+    def search_interval(self, limit: int, first: int, last: int, hit_cap: int = 4096):
+        total = ctypes.c_uint64()
+        hits = (Hit * max(hit_cap, 1))()
+        nh = ctypes.c_uint32()
+        _check(_lib.hss_search_interval(self._h, limit, first, last, ctypes.byref(total), hits, hit_cap,
+                                        ctypes.byref(nh)))
+        return total.value, [(hits[i].l, hits[i].m, hits[i].n, hits[i].area) for i in range(nh.value)]
+
+
+def visible_device() -> int:
+    return int(os.environ.get("HSS_DEVICE", "0"))
