@@ -41,6 +41,43 @@ SLICE_CANDIDATES = 39_091_522_776  # evidence/benchmarks/gpu-opt-benchmark.json:
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # lane-ops per second, in T
 
 
+def profiled_traffic(leaves: int) -> int | None:
+    """HBM bytes per launch measured by the committed rocprofv3 counter passes
+    (profiles/cell_kernel_uneven1536_counters.md), when they describe this
+    exact launch (same leaf count); None otherwise."""
+    path = ROOT / "profiles" / "cell_kernel_uneven1536_counters.md"
+    if not path.is_file():
+        return None
+    kv = {}
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_") and ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = int(v.strip())
+    if kv.get("traffic_leaves_per_launch") != leaves:
+        return None
+    return kv.get("traffic_bytes_per_launch")
+
+
+def plain_form(ctx, roots: list[int]) -> dict:
+    """The reference's scheduling shape: one launch per contiguous run of at
+    most 512 roots (src/gpu_search_v42.cu:456-460), synchronised after each."""
+    runs, i = [], 0
+    while i < len(roots):
+        j = i
+        while j + 1 < len(roots) and roots[j + 1] == roots[j] + 1 and j + 1 - i < 512:
+            j += 1
+        runs.append((roots[i], roots[j]))
+        i = j + 1
+    t0 = time.perf_counter()
+    total = 0
+    for a, b in runs:
+        c, _ = ctx.search_interval(LIMIT, a, b)
+        total += c
+    wall = time.perf_counter() - t0
+    return {"value": len(roots) / wall, "unit": "positions/s", "launches": len(runs), "seconds": wall,
+            "candidates": total}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -81,6 +118,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="uneven1536", choices=["uneven1536", "slice512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain", action="store_true",
+                    help="also time the one-interval-per-launch form of the same roots")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -150,11 +189,19 @@ def main() -> int:
                                "(slice 7500000-7500511 + 512 heavy + 512 light roots, seed 20261203)",
                    "limit": LIMIT, "positions": len(roots), "parallelism": "1 GPU, persistent grid"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": profiled_traffic(leaves),
                      "kernel": "cell_kernel", "ops_per_launch": ops, "kernel_ms": cell_s * 1e3},
         "work": {"leaves": leaves, "candidates": cands, "tiles": stats["tiles"], "deferred": stats["deferred"],
                  "exact_ms": sum(exact_ms) / len(exact_ms), "leaves_per_s": leaves / cell_s},
     }
+    if args.plain:
+        log("timing the plain one-interval-per-launch form")
+        pf = plain_form(ctx, sorted(roots))
+        if pf["candidates"] != ref_total:
+            log(f"PARITY FAILURE: plain form candidates {pf['candidates']} != {ref_total}")
+            return 1
+        pf["batched_over_plain"] = value / pf["value"]
+        line["plain_form"] = pf
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         line["cpu_baseline"] = cpu_baseline(per_root_leaves, sum(per_root_leaves[l] for l in roots), len(roots))

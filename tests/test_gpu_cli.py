@@ -1,0 +1,163 @@
+"""Drop-in contract of the executables on the GPU.
+
+Follows scripts/smoke_test.py:8-61 (cases, argv shapes, STATE fields, exact
+results text), the checkpoint chain rule of scripts/coverage_audit.py:48-68 and
+scripts/gpu_queue.py:40-63 (PROGRESS resume steps of exactly 512, or the final
+short batch, from FIRST to LAST+1), resume from STATE, SIGTERM between
+batches, de-duplication against an existing results file, and the `search`
+front end's pairs counter and coverage skip (src/search_v18.cpp:254-283).
+"""
+from __future__ import annotations
+
+import json
+import os
+import signal
+import subprocess
+import time
+
+import pytest
+
+from conftest import PKG, result_lines
+
+pytestmark = pytest.mark.gpu
+
+GPU = PKG / "gpu_search"
+CPU = PKG / "search"
+CASES = {
+    "full5000": dict(limit=5000, first=1, last=5000, pairs=12502500, candidates=151842556,
+                     results="1853 4380 4427 32918611718880\n"),
+    "known2": dict(limit=70000, first=11789, last=11789, pairs=58212, candidates=1194756,
+                   results="11789 68104 68595 284239560530875680\n"),
+}
+
+
+def read_state(path):
+    return {k: int(v) for k, v in (ln.split() for ln in path.read_text().splitlines())}
+
+
+def chain_end(log_text: str, first: int, last: int):
+    """resume reached by an unbroken +512 PROGRESS chain, or None if broken."""
+    expected = first
+    for line in log_text.splitlines():
+        f = line.split()
+        if len(f) < 3 or f[:2] != ["PROGRESS", "resume"]:
+            continue
+        cur = int(f[2])
+        if cur <= expected:
+            continue
+        if cur != min(expected + 512, last + 1):
+            return None
+        expected = cur
+    return expected
+
+
+def run(cmd, cwd, **kw):
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(cwd / "no-coverage.json"))
+    return subprocess.run([str(c) for c in cmd], cwd=cwd, env=env, capture_output=True, text=True, timeout=600,
+                          **kw)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_search_smoke_cases(tmp_path, name):
+    c = CASES[name]
+    state, results = tmp_path / "gpu-0x-state.txt", tmp_path / "gpu-0x-results.txt"
+    r = run([GPU, c["limit"], c["first"], c["last"], results, 0, state], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st == {"version": 1, "limit": c["limit"], "begin": c["first"], "end": c["last"],
+                  "resume": c["last"] + 1, "candidates": c["candidates"]}
+    assert results.read_text() == c["results"]
+    assert chain_end(r.stdout, c["first"], c["last"]) == c["last"] + 1
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_search_cli_smoke_cases(tmp_path, name):
+    c = CASES[name]
+    state, results = tmp_path / "state.txt", tmp_path / "results.txt"
+    r = run([CPU, c["limit"], c["first"], c["last"], 2, state, results], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st["limit"] == c["limit"] and st["begin"] == c["first"] and st["end"] == c["last"]
+    assert st["resume"] == c["last"] + 1
+    assert st["pairs"] == c["pairs"] and st["candidates"] == c["candidates"]
+    assert results.read_text() == c["results"]
+    assert "HIT " + c["results"].split()[0] in r.stdout
+
+
+def test_resume_from_checkpoint(tmp_path):
+    limit, first, last = 100000, 1, 2000
+    full = tmp_path / "full"
+    full.mkdir()
+    r = run([GPU, limit, first, last, full / "r.txt", 0, full / "s.txt"], full)
+    assert r.returncode == 0, r.stderr
+    total = read_state(full / "s.txt")["candidates"]
+    head = tmp_path / "head"
+    head.mkdir()
+    r = run([GPU, limit, 1, 1024, head / "r.txt", 0, head / "s.txt"], head)
+    part1 = read_state(head / "s.txt")["candidates"]
+    # a checkpoint as a killed worker leaves it after two batches
+    resumed = tmp_path / "resumed"
+    resumed.mkdir()
+    (resumed / "s.txt").write_text(f"version 1\nlimit {limit}\nbegin 1\nend {last}\nresume 1025\ncandidates 7\n")
+    r = run([GPU, limit, first, last, resumed / "r.txt", 0, resumed / "s.txt"], resumed)
+    assert r.returncode == 0, r.stderr
+    st = read_state(resumed / "s.txt")
+    assert st["begin"] == 1025 and st["resume"] == last + 1
+    assert part1 + st["candidates"] == total
+    assert chain_end(r.stdout, 1025, last) == last + 1
+
+
+def test_checkpoint_mismatch_exits_zero(tmp_path):
+    (tmp_path / "s.txt").write_text("version 1\nlimit 5000\nbegin 1\nend 10\nresume 3\ncandidates 0\n")
+    r = run([GPU, 5000, 1, 20, tmp_path / "r.txt", 0, tmp_path / "s.txt"], tmp_path)
+    assert r.returncode == 0 and "checkpoint does not match requested range" in r.stderr
+    assert read_state(tmp_path / "s.txt")["end"] == 10
+
+
+def test_sigterm_between_batches_then_resume(tmp_path):
+    limit, first, last = 1_000_000, 1, 12_288  # 24 batches
+    state, results, log = tmp_path / "gpu-0x-state.txt", tmp_path / "gpu-0x-results.txt", tmp_path / "gpu-0x.log"
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(tmp_path / "none.json"))
+    with log.open("w") as out:
+        p = subprocess.Popen([str(GPU), str(limit), str(first), str(last), str(results), "0", str(state)],
+                             cwd=tmp_path, env=env, stdout=out, stderr=subprocess.STDOUT)
+        deadline = time.time() + 120
+        while time.time() < deadline and "PROGRESS" not in log.read_text():
+            time.sleep(0.05)
+        p.send_signal(signal.SIGTERM)
+        assert p.wait(timeout=120) == 0
+    stopped = read_state(state)["resume"]
+    assert (stopped - first) % 512 == 0 or stopped == last + 1
+    with log.open("a") as out:  # the supervisor appends the relaunch to the same log
+        r = subprocess.run([str(GPU), str(limit), str(first), str(last), str(results), "0", str(state)],
+                           cwd=tmp_path, env=env, stdout=out, stderr=subprocess.STDOUT, timeout=600)
+    assert r.returncode == 0
+    assert read_state(state)["resume"] == last + 1
+    assert chain_end(log.read_text(), first, last) == last + 1
+
+
+def test_results_dedup(tmp_path):
+    c = CASES["known2"]
+    results = tmp_path / "r.txt"
+    results.write_text(c["results"])
+    r = run([GPU, c["limit"], c["first"], c["last"], results, 0, tmp_path / "s.txt"], tmp_path)
+    assert r.returncode == 0
+    assert results.read_text() == c["results"]
+
+
+def test_search_coverage_skip(tmp_path):
+    cov = tmp_path / "cov.json"
+    cov.write_text(json.dumps({"limit": 5000, "complete_intervals": [[1, 4000]], "root_coverage": {}}))
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(cov))
+    r = subprocess.run([str(CPU), "5000", "3990", "4010", "1", str(tmp_path / "s"), str(tmp_path / "r")],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    st = read_state(tmp_path / "s")
+    ref = subprocess.run([str(CPU), "5000", "4001", "4010", "1", str(tmp_path / "s2"), str(tmp_path / "r2")],
+                         cwd=tmp_path, env=dict(os.environ, HERON_COVERAGE_FILE=str(tmp_path / "none")),
+                         capture_output=True, text=True, timeout=600)
+    assert ref.returncode == 0
+    # covered roots count as done (pairs) but contribute no candidates
+    assert st["candidates"] == read_state(tmp_path / "s2")["candidates"]
+    assert st["pairs"] == sum(5000 - l + 1 for l in range(3990, 4011))
+    assert result_lines((tmp_path / "r").read_text()) == []
