@@ -123,7 +123,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(root positions shard with no data exchange; see DESIGN.md)", file=sys.stderr)
+              "(root positions shard across GPUs with no data exchange)", file=sys.stderr)
         return 2
 
     import hss
