@@ -337,6 +337,40 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, uint32_t lef
     }
 }
 
+// One leaf z/2 = X + iY of the fast path: the arithmetic of eval_leaf, with the
+// second orientation evaluated only when some lane's area test for it passes
+// (for l, m both odd it never does: Re z is odd, so 2 Re z != 0 mod 8).  Adds
+// the decided count to cand, sends an undecided leaf to the exact kernel and
+// queues counted terms for the root-proximity test.  kc = k | conj << 30.
+__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, uint32_t left, bool live, double X, double Y,
+                                 uint32_t vX, uint32_t vY, const LaneCtx& c, uint32_t ui, uint32_t vidx, uint32_t kc,
+                                 uint64_t& cand) {
+    const double ax = X < 0 ? -X : X, ay = Y < 0 ? -Y : Y;
+    const bool a1 = live && area_ok(vY, c);
+    bool t1, t2, u1;
+    range_terms(X, c, t1, t2, u1);
+    bool und = a1 && (ay <= c.tau || u1);
+    bool f0 = a1 && t1, f1 = a1 && t2, f2 = false, f3 = false;
+    const bool a2 = live && area_ok(vX, c);
+    if (__ballot(a2)) {
+        const double dm = X - Y, dp = X + Y;
+        const bool eq = (dm < 0 ? -dm : dm) <= c.tau || (dp < 0 ? -dp : dp) <= c.tau;
+        bool t3, t4, u2;
+        range_terms(Y, c, t3, t4, u2);
+        const bool o2 = a2 && !eq;
+        und = und || (a2 && (eq || ax <= c.tau)) || (o2 && u2);
+        f2 = o2 && t3;
+        f3 = o2 && t4;
+    }
+    cand += und ? 0u : (uint32_t)f0 + (uint32_t)f1 + (uint32_t)f2 + (uint32_t)f3;
+    push_global(P, und, DefItem{left, ui, vidx, kc | 0x80000000u});
+    const bool ok = !und;
+    root_push(P, rq, left, ok && f0, c.lm - 2.0 * ax, ui, vidx, kc);
+    root_push(P, rq, left, ok && f1, c.lm + 2.0 * ax, ui, vidx, kc);
+    root_push(P, rq, left, ok && f2, c.lm - 2.0 * ay, ui, vidx, kc);
+    root_push(P, rq, left, ok && f3, c.lm + 2.0 * ay, ui, vidx, kc);
+}
+
 __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
     __shared__ double s_qz[WAVES][QCAP];
     __shared__ uint4 s_qid[WAVES][QCAP];
@@ -387,40 +421,31 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
                     LaneCtx ctx;
                     make_lane_ctx(m, S.l4_d, S.l2_d, P.n4_d, S.l3, S.l5, m3, m5, ctx);
                     leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
-                    for (uint32_t ui = 0; ui < L.H; ++ui) {
+                    // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
+                    for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
                         const LeftFast u = P.lfast[L.off + ui];
-                        const bool uc = ui + 1 == L.H;
                         const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
                         const double t1 = u.im * d, t2 = u.re * d;
                         const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
-                        for (uint32_t cj = 0; cj < 2; ++cj) {
-                            if (cj == 1 && uc) break;
-                            const bool live = act && (cj == 0 || !vc);
-                            double X, Y;
-                            uint32_t vX, vY;
-                            if (cj == 0) {
-                                X = __builtin_fma(u.re, c, -t1);
-                                Y = __builtin_fma(u.im, c, t2);
-                                vX = pac + QA * QA - pbd;
-                                vY = pad + pbc;
-                            } else {
-                                X = __builtin_fma(u.re, c, t1);
-                                Y = __builtin_fma(u.im, c, -t2);
-                                vX = pac + pbd;
-                                vY = pbc + QA * QA - pad;
-                            }
-
-                            // both evaluations, then a select: no per-lane branch here
-                            LeafEval ra, rb;
-                            eval_leaf(X, Y, vX, vY, ctx, ra);
-                            eval_real_leaf(vX, ctx, rb);  // both factors real
-                            const LeafEval r = select_leaf(cj == 0 && uc && vc, rb, ra);
-                            cand += (live && !r.und) ? r.cnt : 0u;
-                            const uint32_t kc = kj | cj << 30;
-                            push_global(P, live && r.und, DefItem{T.left, ui, vidx, kc | 0x80000000u});
-                            const bool ok = live && !r.und;
-                            for (int tm = 0; tm < 4; ++tm) root_push(P, rq, T.left, ok && r.f[tm], r.zz[tm], ui, vidx, kc);
-                        }
+                        leaf_fast(P, rq, T.left, act, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
+                                  pac + QA * QA - pbd, pad + pbc, ctx, ui, vidx, kj, cand);
+                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, t1), __builtin_fma(u.im, c, -t2),
+                                  pac + pbd, pbc + QA * QA - pad, ctx, ui, vidx, kj | 1u << 30, cand);
+                    }
+                    // the central (real) u: leaf u*v only; with a real v too the real-leaf rule
+                    {
+                        const uint32_t ui = L.H - 1;
+                        const LeftFast u = P.lfast[L.off + ui];
+                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                        const double t1 = u.im * d, t2 = u.re * d;
+                        const uint32_t vX = ua * c180 + QA * QA - ub * d180, vY = ua * d180 + ub * c180;
+                        LeafEval rr;
+                        eval_real_leaf(vX, ctx, rr);
+                        const bool real = act && vc;
+                        cand += (real && !rr.und) ? rr.cnt : 0u;
+                        push_global(P, real && rr.und, DefItem{T.left, ui, vidx, kj | 0x80000000u});
+                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
+                                  vX, vY, ctx, ui, vidx, kj, cand);
                     }
                 }
             }
