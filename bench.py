@@ -19,6 +19,11 @@ the kernel's HIP-event duration; peak = 256 CU x 64 lanes x 2.4 GHz.
 cpu_baseline: the reference search_v18 compiled from source (oracle/_ref/search),
 timed on this host on a bounded sample of the same workload and scaled by the
 exact leaf counts to the whole batch.
+served: the seeded request stream of workloads.served_stream (40 interval
+requests, LIMIT in {1e5, 1e6, 1e7}, length log-uniform in [1, 4096], Poisson
+arrivals at 5 req/s) submitted in real time to the request server; latency is
+submit -> results ready per request (p50 / p99 by linear interpolation), and
+every served result is compared with the same request searched alone.
 """
 from __future__ import annotations
 
@@ -78,6 +83,49 @@ def plain_form(ctx, roots: list[int]) -> dict:
             "candidates": total}
 
 
+def served_leg(ctx, n_req: int, rate_hz: float) -> dict:
+    """Run the seeded request stream through the server in real time, then
+    search each request alone and compare (bit-exact)."""
+    import numpy as np
+    import hss
+    import workloads
+
+    stream = workloads.served_stream(n_req, rate_hz)
+    ids = []
+    with hss.Server(ctx) as srv:
+        t0 = time.perf_counter()
+        for at, limit, first, last in stream:
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(limit, first, last))
+        served = [srv.result(i) for i in ids]
+        st = srv.stats()
+    lat = np.array([r.info["done_ms"] - r.info["submit_ms"] for r in served])
+    span_ms = max(r.info["done_ms"] for r in served) - min(r.info["submit_ms"] for r in served)
+    positions = sum(last - first + 1 for _, _, first, last in stream)
+    log(f"served {n_req} requests ({positions} positions) in {span_ms / 1e3:.2f} s; "
+        f"p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms; checking each alone")
+    equal, alone_ms = True, []
+    for (_, limit, first, last), r in zip(stream, served):
+        t1 = time.perf_counter()
+        a = ctx.search(limit, range(first, last + 1))
+        alone_ms.append((time.perf_counter() - t1) * 1e3)
+        if a.candidates != r.candidates or a.leaves != r.leaves or a.pairs != r.pairs or a.hits != r.hits:
+            equal = False
+            log(f"MISMATCH served vs alone for request {limit} {first} {last}")
+    slowdown = lat / np.maximum(np.array(alone_ms), 1e-3)
+    return {"requests": n_req, "positions": positions, "rate_hz": rate_hz, "seed": workloads.SEED,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "throughput_positions_per_s": positions / (span_ms / 1e3),
+            "leaves_per_s": st["leaves"] / (span_ms / 1e3), "span_s": span_ms / 1e3,
+            "gpu_busy_frac": st["gpu_ms"] / span_ms, "launches": st["launches"],
+            "shared_launches": st["shared_launches"], "max_active": st["max_active"],
+            "candidates": st["candidates"], "alone_equal": equal,
+            "alone_sequential_s": sum(alone_ms) / 1e3, "median_slowdown_vs_alone": float(np.median(slowdown))}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -120,6 +168,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain", action="store_true",
                     help="also time the one-interval-per-launch form of the same roots")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request stream")
+    ap.add_argument("--served-requests", type=int, default=40)
+    ap.add_argument("--served-rate", type=float, default=5.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -202,6 +253,14 @@ def main() -> int:
             return 1
         pf["batched_over_plain"] = value / pf["value"]
         line["plain_form"] = pf
+    if not args.no_served:
+        log(f"served stream: {args.served_requests} requests at {args.served_rate} req/s")
+        sv = served_leg(ctx, args.served_requests, args.served_rate)
+        if not sv["alone_equal"]:
+            log("PARITY FAILURE: served results differ from the alone searches")
+            return 1
+        line["served"] = sv
+        line["latency_ms"] = {"p50": sv["p50_ms"], "p99": sv["p99_ms"]}
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         line["cpu_baseline"] = cpu_baseline(per_root_leaves, sum(per_root_leaves[l] for l in roots), len(roots))

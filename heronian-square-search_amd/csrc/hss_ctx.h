@@ -1,0 +1,129 @@
+// hss_ctx.h -- internal state of a context and the per-launch buffer sets
+// shared by the synchronous search (hss_runtime.cpp) and the request server
+// (hss_server.cpp).  Host code only; no part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hss.h"
+#include "hss_kernels.h"
+#include "hss_plan.h"
+
+namespace hss {
+
+// thread-local error text behind hss_last_error()
+int fail(int code, const std::string& msg);
+
+#define HSS_HIP_TRY(x)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return ::hss::fail(e_ == hipErrorOutOfMemory ? HSS_ENOMEM : HSS_EDEVICE,       \
+                               std::string(#x) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want);
+    void release();
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// grow-only pinned host buffer (allocated pinned: no registration of existing memory)
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want);
+    void release();
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+// Counter arrays of one launch target: cand[i], leaves[i], pairs[i] for every
+// counter index i < cap (PosDesc::out).
+struct Counters {
+    unsigned long long* cand = nullptr;
+    unsigned long long* leaves = nullptr;
+    unsigned long long* pairs = nullptr;
+};
+
+// Everything one launch of the search needs besides the resident tables.  A
+// buffer set is reused only after its launch has completed.
+struct LaunchBuffers {
+    DevBuf pos, lefts, builds, tiles, lfast, lre, lim, misc, def, raw, hits;
+    HostBuf upload;   // pinned staging of the plan
+    HostBuf h_misc;   // pinned: [0] tile head, [1] deferred, [2] raw, [3] hits
+    uint32_t def_cap = 0;
+    hipEvent_t ev[4] = {};  // start, setup done, cells done, end
+    Plan plan;
+    uint32_t npos = 0;
+    static constexpr uint32_t RAW_CAP = 1u << 16, HIT_CAP = 1u << 16;
+    hipError_t create();
+    void release();
+};
+
+// Outcome of a completed launch.
+struct LaunchResult {
+    bool rerun = false;        // deferred buffer overflowed: grown, launch again
+    uint32_t deferred = 0;
+    uint32_t raw = 0;
+    std::vector<HitOut> hits;  // accepted (not de-duplicated), HitOut::pos = plan position
+    float setup_ms = 0, cells_ms = 0, exact_ms = 0;
+};
+
+}  // namespace hss
+
+struct hss_server;
+
+struct hss_ctx {
+    int device = 0;
+    uint64_t nmax = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    int blocks_per_cu = 1;
+    uint64_t device_bytes = 0;
+    std::vector<uint32_t> small_primes;
+    std::vector<uint32_t> h_off;  // half-set offsets, nmax + 2 entries
+    uint32_t cache_total = 0;
+    hss::QR4 qr{};
+    // resident tables
+    uint32_t* d_off = nullptr;
+    double* d_cre = nullptr;
+    double* d_cim = nullptr;
+    uint32_t* d_cres = nullptr;
+    uint32_t* d_vbase = nullptr;
+    hss::i128* d_vre = nullptr;
+    hss::i128* d_vim = nullptr;
+    // synchronous search path
+    hss::LaunchBuffers sync;
+    hss::DevBuf counters;
+    hss_stats stats{};
+    hss_server* server = nullptr;  // a server drives the context while attached
+
+    uint64_t waves() const { return (uint64_t)cus * blocks_per_cu * 4; }
+    uint32_t grid_blocks() const { return (uint32_t)(cus * blocks_per_cu); }
+};
+
+namespace hss {
+
+// Plan `n` positions into B.plan (host only).
+int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n);
+// Enqueue the planned launch on stream st: uploads, left-set build, reset of
+// the positions' counters, cells, exact and acceptance kernels, read-back of
+// the misc counters into pinned memory.  Asynchronous.
+int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
+// After B.ev[3] has completed: read the misc counters, detect overflow, copy
+// accepted hits on `copy` (a stream with no work that writes B's buffers).
+int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out);
+
+}  // namespace hss

@@ -30,17 +30,15 @@ struct CellParams {
     const i128* lre;
     const i128* lim;
     uint32_t* next_tile;
-    unsigned long long* cand;    // per position
-    unsigned long long* leaves;  // per position
-    unsigned long long* pairs;   // per position: pairs reaching the enumeration
+    unsigned long long* cand;    // per counter index (PosDesc::out)
+    unsigned long long* leaves;  // per counter index
+    unsigned long long* pairs;   // per counter index: pairs reaching the enumeration
     DefItem* def;     // leaves for the exact kernel
     uint32_t* ndef;
     uint32_t def_cap;
     HitOut* raw;      // exact fourth powers before the acceptance rules
     uint32_t* nraw;
     uint32_t raw_cap;
-    uint64_t limit;
-    double n4_d;
     QR4 qr;
 };
 
@@ -57,9 +55,11 @@ hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uin
                              const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast,
                              hipStream_t st);
 hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st);
-hipError_t launch_exact(const CellParams& P, uint32_t n, hipStream_t st);
-hipError_t launch_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+hipError_t launch_exact(const CellParams& P, uint32_t blocks, hipStream_t st);
+hipError_t launch_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
                          uint32_t* nhits, uint32_t hit_cap, hipStream_t st);
+hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand, unsigned long long* leaves,
+                              unsigned long long* pairs, uint32_t* misc, hipStream_t st);
 hipError_t cell_occupancy(int* blocks_per_cu);
 
 }  // namespace hss

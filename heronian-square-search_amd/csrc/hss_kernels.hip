@@ -284,17 +284,6 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
     if (pred && slot < P.def_cap) P.def[slot] = it;
 }
 
-__device__ inline LeafEval select_leaf(bool c, const LeafEval& a, const LeafEval& b) {
-    LeafEval r;
-    r.cnt = c ? a.cnt : b.cnt;
-    r.und = c ? a.und : b.und;
-    for (int i = 0; i < 4; ++i) {
-        r.f[i] = c ? a.f[i] : b.f[i];
-        r.zz[i] = c ? a.zz[i] : b.zz[i];
-    }
-    return r;
-}
-
 // Per-wave LDS queue of counted terms awaiting the root-proximity test.
 struct RootQueue {
     double* z;
@@ -419,7 +408,7 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
                     uint64_t m = (uint64_t)L.q * kj;
                     uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
                     LaneCtx ctx;
-                    make_lane_ctx(m, S.l4_d, S.l2_d, P.n4_d, S.l3, S.l5, m3, m5, ctx);
+                    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, ctx);
                     leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
                     // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
                     for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
@@ -460,15 +449,18 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
         leaves = wave_sum64(leaves);
         pairs = wave_sum64(pairs);
         // all lanes execute the atomics; only lane 0 contributes
-        atomicAdd(P.cand + L.pos, lane == 0 ? (unsigned long long)cand : 0ull);
-        atomicAdd(P.leaves + L.pos, lane == 0 ? (unsigned long long)leaves : 0ull);
-        atomicAdd(P.pairs + L.pos, lane == 0 ? (unsigned long long)pairs : 0ull);
+        atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
+        atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
+        atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
     }
 }
 
 // Exact settlement of deferred leaves: full reference chain with 128-bit
 // integers; counts are added only for leaves the fast path left undecided.
-__global__ void k_exact(const CellParams P, uint32_t n) {
+// The item count is read on the device (an overflowed buffer holds def_cap
+// items; the host then runs the batch again with a larger buffer).
+__global__ void k_exact(const CellParams P) {
+    const uint32_t n = *P.ndef < P.def_cap ? *P.ndef : P.def_cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const DefItem it = P.def[i];
         const LeftDesc L = P.lefts[it.left];
@@ -477,29 +469,45 @@ __global__ void k_exact(const CellParams P, uint32_t n) {
         const bool conj = (it.kflags >> 30) & 1u, full = it.kflags >> 31;
         const uint64_t m = (uint64_t)L.q * k;
         ExactCtx e;
-        make_exact_ctx(S.l, m, P.limit, e);
+        make_exact_ctx(S.l, m, S.limit, e);
         i128 zre, zim;
         exact_product(P.lre[L.off + it.ui], P.lim[L.off + it.ui], (int64_t)P.cre[it.vidx], (int64_t)P.cim[it.vidx],
                       conj, zre, zim);
         uint32_t c = exact_leaf(zre, zim, e, P.qr, [&](uint64_t r, u128 a4) {
             uint32_t slot = atomicAdd(P.nraw, 1u);
-            if (slot < P.raw_cap) P.raw[slot] = HitOut{S.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64)};
+            if (slot < P.raw_cap) P.raw[slot] = HitOut{S.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64), L.pos, 0u};
         });
-        if (full && c) atomicAdd(P.cand + L.pos, (unsigned long long)c);
+        if (full && c) atomicAdd(P.cand + S.out, (unsigned long long)c);
     }
 }
 
 // Acceptance of the raw exact fourth powers: gcd, 5 | lmn, ordering and the
-// 256-bit Heron recheck; accepted triples are appended to the hit list.
-__global__ void k_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+// 256-bit Heron recheck against the position's own LIMIT; accepted triples
+// are appended to the hit list.
+__global__ void k_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
                          uint32_t* nhits, uint32_t hit_cap) {
     uint32_t n = *nraw < raw_cap ? *nraw : raw_cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         HitOut h = raw[i];
         u128 a4 = ((u128)h.area4_hi << 64) | h.area4_lo;
-        if (!accept_hit(h.l, h.m, h.n, limit, a4)) continue;
+        if (!accept_hit(h.l, h.m, h.n, pos[h.pos].limit, a4)) continue;
         uint32_t slot = atomicAdd(nhits, 1u);
         if (slot < hit_cap) hits[slot] = h;
+    }
+}
+
+// Start of a batch: zero the work-queue head and the deferred / raw / hit
+// counters (misc[0..3]) and the counters of the batch's positions.  A kernel
+// rather than memset nodes, so the launch sequence can be captured in a graph.
+__global__ void k_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand,
+                              unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) misc[i] = 0;
+    for (; i < npos; i += gridDim.x * blockDim.x) {
+        const uint32_t o = pos[i].out;
+        cand[o] = 0;
+        leaves[o] = 0;
+        pairs[o] = 0;
     }
 }
 
@@ -570,17 +578,22 @@ hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_exact(const CellParams& P, uint32_t n, hipStream_t st) {
-    if (!n) return hipSuccess;
-    uint32_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_exact, dim3(blocks), dim3(256), 0, st, P, n);
+hipError_t launch_exact(const CellParams& P, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_exact, dim3(blocks), dim3(256), 0, st, P);
     return hipGetLastError();
 }
 
-hipError_t launch_accept(const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, uint64_t limit, HitOut* hits,
+hipError_t launch_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
                          uint32_t* nhits, uint32_t hit_cap, hipStream_t st) {
-    hipLaunchKernelGGL(k_accept, dim3(16), dim3(256), 0, st, raw, nraw, raw_cap, limit, hits, nhits, hit_cap);
+    hipLaunchKernelGGL(k_accept, dim3(16), dim3(256), 0, st, pos, raw, nraw, raw_cap, hits, nhits, hit_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand, unsigned long long* leaves,
+                              unsigned long long* pairs, uint32_t* misc, hipStream_t st) {
+    uint32_t blocks = (npos + 255) / 256;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, st, pos, npos, cand, leaves, pairs, misc);
     return hipGetLastError();
 }
 

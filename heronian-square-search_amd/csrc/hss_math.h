@@ -326,6 +326,8 @@ HSS_HD inline bool accept_hit(uint64_t l, uint64_t m, uint64_t n, uint64_t limit
 struct HitOut {
     uint64_t l, m, n;
     uint64_t area4_lo, area4_hi;  // |4A|
+    uint32_t pos;                 // position index in the batch that found it
+    uint32_t pad;
 };
 
 // Full reference filter chain of one leaf z (exact).  Returns the candidates;

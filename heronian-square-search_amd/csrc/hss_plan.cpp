@@ -67,18 +67,65 @@ bool factor_root(uint64_t l, const std::vector<uint32_t>& primes, RootFactors& f
 // partners' half-set entries with index in [a, b]
 inline uint64_t entries(const uint32_t* off, uint32_t a, uint32_t b) { return (uint64_t)off[b + 1] - off[a]; }
 
+// Visit every signature of root l: exponent vectors s_j >= 0 over l's split
+// primes with q = prod p_j^s_j <= limit.  visit(q, sexp) is called once each.
+template <typename Visit>
+void for_each_signature(const RootFactors& f, uint64_t limit, Visit&& visit) {
+    uint32_t sexp[MAX_SPLIT] = {0};
+    uint64_t qpow[MAX_SPLIT + 1];
+    for (uint32_t j = 0; j <= f.ns; ++j) qpow[j] = 1;
+    for (;;) {
+        visit(qpow[f.ns], sexp);
+        // increment: last position first
+        int j = (int)f.ns - 1;
+        while (j >= 0) {
+            uint64_t nq = qpow[j + 1] * f.sp[j];
+            if (nq <= limit) {
+                ++sexp[j];
+                qpow[j + 1] = nq;
+                for (uint32_t t = (uint32_t)j + 1; t < f.ns; ++t) {
+                    sexp[t] = 0;
+                    qpow[t + 1] = qpow[t];
+                }
+                break;
+            }
+            --j;
+        }
+        if (j < 0) break;
+    }
+}
+
+uint64_t left_size(const RootFactors& f, const uint32_t* sexp) {
+    uint64_t W = 1;
+    for (uint32_t j = 0; j < f.ns; ++j) W *= 4ull * (f.se[j] + sexp[j]) + 1;
+    return (W + 1) / 2;
+}
+
 }  // namespace
 
-bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const std::vector<uint32_t>& small_primes,
-                const uint32_t* off, uint64_t n_max, uint64_t waves, Plan& P, std::string& err) {
+double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                         uint64_t n_max) {
+    if (limit < 1 || limit > n_max || l < 1 || l > limit) return -1.0;
+    RootFactors f;
+    if (!factor_root(l, small_primes, f)) return -1.0;
+    double est = 0;
+    for_each_signature(f, limit, [&](uint64_t q, const uint32_t* sexp) {
+        uint64_t kmin = (l + q - 1) / q, kmax = limit / q;
+        if (kmin <= kmax) est += (double)left_size(f, sexp) * (double)entries(off, (uint32_t)kmin, (uint32_t)kmax);
+    });
+    return est;
+}
+
+bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                uint64_t n_max, uint64_t waves, Plan& P, std::string& err) {
     P = Plan();
-    if (limit < 1 || limit > n_max) {
-        err = "limit outside [1, n_max]";
-        return false;
-    }
     P.pos.resize(n_roots);
     for (uint32_t s = 0; s < n_roots; ++s) {
-        const uint64_t l = roots[s];
+        const uint64_t l = reqs[s].l, limit = reqs[s].limit;
+        if (limit < 1 || limit > n_max) {
+            err = "limit outside [1, n_max]";
+            return false;
+        }
         if (l < 1 || l > limit) {
             err = "root outside [1, limit]";
             return false;
@@ -91,7 +138,10 @@ bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const s
         PosDesc& D = P.pos[s];
         D = PosDesc();
         D.l = l;
-        double l2 = (double)(l * l);
+        D.limit = limit;
+        const double nd = (double)(limit * limit);
+        D.n4_d = nd * nd;
+        const double l2 = (double)(l * l);
         D.l2_d = l2;
         D.l4_d = l2 * l2;
         D.l3 = (uint32_t)(l % 3);
@@ -99,6 +149,7 @@ bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const s
         D.l_even = (l & 1) ? 0u : 1u;
         D.nsplit = f.ns;
         D.ninert = f.ni;
+        D.out = reqs[s].out;
         for (uint32_t j = 0; j < f.ns; ++j) {
             D.split_inv[j] = inv_mod_2_32(f.sp[j]);
             D.split_thr[j] = 0xFFFFFFFFu / f.sp[j];
@@ -107,24 +158,20 @@ bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const s
             D.inert_inv[j] = inv_mod_2_32(f.ip[j]);
             D.inert_thr[j] = 0xFFFFFFFFu / f.ip[j];
         }
-        // signatures: exponent vectors s_j >= 0 with q = prod p_j^s_j <= limit
-        uint32_t sexp[MAX_SPLIT] = {0};
-        auto emit = [&](uint64_t q) {
+        for_each_signature(f, limit, [&](uint64_t q, const uint32_t* sexp) {
             uint64_t kmin = (l + q - 1) / q, kmax = limit / q;
             if (kmin > kmax) return;
             LeftBuild B{};
             B.fixed = f.fixed;
             B.k = f.ns;
-            uint64_t W = 1;
             for (uint32_t j = 0; j < f.ns; ++j) {
                 B.p[j] = f.sp[j];
                 B.E[j] = f.se[j] + sexp[j];
-                W *= 4ull * B.E[j] + 1;
             }
             LeftDesc L{};
             L.pos = s;
             L.off = (uint32_t)P.left_entries;
-            L.H = (uint32_t)((W + 1) / 2);
+            L.H = (uint32_t)left_size(f, sexp);
             L.q = (uint32_t)q;
             L.trivial = f.ns == 0 ? 1u : 0u;
             L.q3 = (uint32_t)(q % 3);
@@ -135,31 +182,7 @@ bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const s
             P.kmin.push_back((uint32_t)kmin);
             P.kmax.push_back((uint32_t)kmax);
             P.est_cells += (double)L.H * (double)entries(off, (uint32_t)kmin, (uint32_t)kmax);
-        };
-        // iterative odometer over exponent vectors
-        uint64_t qpow[MAX_SPLIT + 1];
-        qpow[0] = 1;
-        std::fill(sexp, sexp + MAX_SPLIT, 0u);
-        for (uint32_t j = 0; j < f.ns; ++j) qpow[j + 1] = qpow[j];
-        for (;;) {
-            emit(qpow[f.ns]);
-            // increment: last position first
-            int j = (int)f.ns - 1;
-            while (j >= 0) {
-                uint64_t nq = qpow[j + 1] * f.sp[j];
-                if (nq <= limit) {
-                    ++sexp[j];
-                    qpow[j + 1] = nq;
-                    for (uint32_t t = (uint32_t)j + 1; t < f.ns; ++t) {
-                        sexp[t] = 0;
-                        qpow[t + 1] = qpow[t];
-                    }
-                    break;
-                }
-                --j;
-            }
-            if (j < 0) break;
-        }
+        });
     }
     if (P.left_entries > 0xFFFFFFFFull) {
         err = "batch too large: split it";

@@ -37,9 +37,24 @@ std::vector<uint32_t> primes_upto(uint32_t upto);
 // Multiplicative inverse of an odd 32-bit value modulo 2^32.
 uint32_t inv_mod_2_32(uint32_t p);
 
+// One root position of a batch: root side l searched against partners and
+// third sides <= limit; `out` indexes the counters the position's counts go to.
+struct PosReq {
+    uint64_t l, limit;
+    uint32_t out;
+};
+
 // Plan one batch.  `off` is the host copy of the half-set offsets (n_max + 2
 // entries); `waves` the number of resident wavefronts used to size tiles.
-bool plan_batch(uint64_t limit, const uint64_t* roots, uint32_t n_roots, const std::vector<uint32_t>& small_primes,
-                const uint32_t* off, uint64_t n_max, uint64_t waves, Plan& out, std::string& err);
+// Positions may repeat (the same l twice, or at two LIMITs): each is its own
+// unit of work.
+bool plan_batch(const PosReq* reqs, uint32_t n, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                uint64_t n_max, uint64_t waves, Plan& out, std::string& err);
+
+// Work estimate of one root position in cells (left entry x partner half-set
+// entry), the same measure plan_batch sizes tiles with; -1 if l cannot be
+// planned (l outside [1, limit], limit > n_max, too many distinct primes).
+double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                         uint64_t n_max);
 
 }  // namespace hss

@@ -1,7 +1,9 @@
-// hss_runtime.cpp -- host runtime behind the C ABI of include/hss.h.
+// hss_runtime.cpp -- host runtime behind the C ABI of include/hss.h:
+// context creation (resident number-theory tables), the launch sequence of one
+// batch of root positions, and the synchronous search entry points.
 //
 // One context = one HIP device, one stream (default priority), the resident
-// number-theory tables, and grow-only scratch buffers for batches.
+// tables, and grow-only buffer sets for launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,13 +17,11 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/hss.h"
-#include "hss_kernels.h"
-#include "hss_plan.h"
+#include "hss_ctx.h"
 
 using namespace hss;
 
-namespace {
+namespace hss {
 
 thread_local std::string g_error;
 
@@ -30,38 +30,63 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(x)                                                                         \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            return fail(e_ == hipErrorOutOfMemory ? HSS_ENOMEM : HSS_EDEVICE,              \
-                        std::string(#x) + ": " + hipGetErrorString(e_));                   \
-    } while (0)
+hipError_t DevBuf::reserve(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t n = std::max<size_t>(want + want / 4, 4096);
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+}
 
-// grow-only device buffer
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t want) {
-        if (want <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t n = std::max<size_t>(want + want / 4, 4096);
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        return e;
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+hipError_t HostBuf::reserve(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t n = std::max<size_t>(want + want / 4, 4096);
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = n;
+    return e;
+}
+
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+hipError_t LaunchBuffers::create() {
+    for (hipEvent_t& e : ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
     }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
+    return h_misc.reserve(64);
+}
+
+void LaunchBuffers::release() {
+    for (DevBuf* b : {&pos, &lefts, &builds, &tiles, &lfast, &lre, &lim, &misc, &def, &raw, &hits}) b->release();
+    upload.release();
+    h_misc.release();
+    for (hipEvent_t& e : ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
     }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
+}
+
+}  // namespace hss
+
+namespace {
+
+#define HIP_TRY HSS_HIP_TRY
 
 // HSS_VERBOSE=1: synchronize and report after each phase (diagnostics only)
 bool verbose() {
@@ -85,33 +110,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 }  // namespace
 
-struct hss_ctx {
-    int device = 0;
-    uint64_t nmax = 0;
-    hipStream_t stream = nullptr;
-    int cus = 0;
-    int blocks_per_cu = 1;
-    uint64_t device_bytes = 0;
-    std::vector<uint32_t> small_primes;
-    std::vector<uint32_t> h_off;  // half-set offsets, nmax + 2 entries
-    uint32_t cache_total = 0;
-    QR4 qr{};
-    // resident tables
-    uint32_t* d_off = nullptr;
-    double* d_cre = nullptr;
-    double* d_cim = nullptr;
-    uint32_t* d_cres = nullptr;
-    uint32_t* d_vbase = nullptr;
-    i128* d_vre = nullptr;
-    i128* d_vim = nullptr;
-    // batch scratch
-    DevBuf pos, lefts, builds, tiles, lfast, lre, lim, counters, misc, def, raw, hits;
-    uint32_t def_cap = 0;
-    hipEvent_t ev[6] = {};
-    hss_stats stats{};
-};
-
-extern "C" const char* hss_last_error(void) { return g_error.c_str(); }
+extern "C" const char* hss_last_error(void) { return hss::g_error.c_str(); }
 
 extern "C" uint64_t hss_ctx_nmax(const hss_ctx* c) { return c ? c->nmax : 0; }
 
@@ -119,6 +118,7 @@ extern "C" uint64_t hss_ctx_device_bytes(const hss_ctx* c) { return c ? c->devic
 
 extern "C" void hss_ctx_destroy(hss_ctx* c) {
     if (!c) return;
+    if (c->server) hss_server_destroy(c->server);  // finishes its requests first
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->d_off);
@@ -128,11 +128,8 @@ extern "C" void hss_ctx_destroy(hss_ctx* c) {
     (void)hipFree(c->d_vbase);
     (void)hipFree(c->d_vre);
     (void)hipFree(c->d_vim);
-    for (DevBuf* b : {&c->pos, &c->lefts, &c->builds, &c->tiles, &c->lfast, &c->lre, &c->lim, &c->counters,
-                      &c->misc, &c->def, &c->raw, &c->hits})
-        b->release();
-    for (hipEvent_t& e : c->ev)
-        if (e) (void)hipEventDestroy(e);
+    c->sync.release();
+    c->counters.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -252,8 +249,7 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
     c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(HSS_EDEVICE, "stream create failed"));
-    for (hipEvent_t& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(HSS_EDEVICE, "event create failed"));
+    if (c->sync.create() != hipSuccess) return bail(fail(HSS_EDEVICE, "launch buffers: event or pinned memory"));
     int bpc = 0;
     if (cell_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
     c->blocks_per_cu = bpc;
@@ -271,124 +267,130 @@ extern "C" int hss_get_stats(const hss_ctx* c, hss_stats* out) {
     return HSS_OK;
 }
 
-static int run_batch(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
-                     std::vector<unsigned long long>& cnt, std::vector<HitOut>& hits_out, uint32_t& raw_count) {
-    auto t0 = std::chrono::steady_clock::now();
-    hipStream_t st = c->stream;
-    Plan P;
+namespace hss {
+
+int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
     std::string err;
-    const uint64_t waves = (uint64_t)c->cus * c->blocks_per_cu * 4;
-    if (!plan_batch(limit, roots, n_roots, c->small_primes, c->h_off.data(), c->nmax, waves, P, err))
+    if (!plan_batch(reqs, n, c->small_primes, c->h_off.data(), c->nmax, c->waves(), B.plan, err))
         return fail(HSS_EINVAL, err);
-    c->stats.plan_ms = ms_since(t0);
-    const uint32_t npos = n_roots, nleft = (uint32_t)P.lefts.size(), ntiles = (uint32_t)P.tiles.size();
-    HIP_TRY(c->pos.reserve(npos * sizeof(PosDesc)));
-    HIP_TRY(c->lefts.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftDesc)));
-    HIP_TRY(c->builds.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftBuild)));
-    HIP_TRY(c->tiles.reserve(std::max<size_t>(ntiles, 1) * sizeof(TileDesc)));
-    HIP_TRY(c->lfast.reserve(std::max<uint64_t>(P.left_entries, 1) * sizeof(LeftFast)));
-    HIP_TRY(c->lre.reserve(std::max<uint64_t>(P.left_entries, 1) * sizeof(i128)));
-    HIP_TRY(c->lim.reserve(std::max<uint64_t>(P.left_entries, 1) * sizeof(i128)));
-    HIP_TRY(c->counters.reserve((size_t)npos * 3 * 8));
-    HIP_TRY(c->misc.reserve(64));
-    const uint32_t raw_cap = 1u << 16, hit_cap = 1u << 16;
-    HIP_TRY(c->raw.reserve(raw_cap * sizeof(HitOut)));
-    HIP_TRY(c->hits.reserve(hit_cap * sizeof(HitOut)));
-    if (!c->def_cap) {
-        c->def_cap = 1u << 22;
-        HIP_TRY(c->def.reserve((size_t)c->def_cap * sizeof(DefItem)));
+    B.npos = n;
+    return HSS_OK;
+}
+
+int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
+    const Plan& P = B.plan;
+    const uint32_t npos = B.npos, nleft = (uint32_t)P.lefts.size(), ntiles = (uint32_t)P.tiles.size();
+    const size_t entries = std::max<uint64_t>(P.left_entries, 1);
+    HIP_TRY(B.pos.reserve(std::max<size_t>(npos, 1) * sizeof(PosDesc)));
+    HIP_TRY(B.lefts.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftDesc)));
+    HIP_TRY(B.builds.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftBuild)));
+    HIP_TRY(B.tiles.reserve(std::max<size_t>(ntiles, 1) * sizeof(TileDesc)));
+    HIP_TRY(B.lfast.reserve(entries * sizeof(LeftFast)));
+    HIP_TRY(B.lre.reserve(entries * sizeof(i128)));
+    HIP_TRY(B.lim.reserve(entries * sizeof(i128)));
+    HIP_TRY(B.misc.reserve(64));
+    HIP_TRY(B.raw.reserve(LaunchBuffers::RAW_CAP * sizeof(HitOut)));
+    HIP_TRY(B.hits.reserve(LaunchBuffers::HIT_CAP * sizeof(HitOut)));
+    if (!B.def_cap) {
+        B.def_cap = 1u << 22;
+        HIP_TRY(B.def.reserve((size_t)B.def_cap * sizeof(DefItem)));
     }
-    HIP_TRY(hipEventRecord(c->ev[0], st));
-    HIP_TRY(hipMemcpyAsync(c->pos.p, P.pos.data(), npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
+    // stage the plan in pinned memory: one region per array, 256-byte aligned
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_pos = al(npos * sizeof(PosDesc)), b_left = al(nleft * sizeof(LeftDesc)),
+                 b_build = al(nleft * sizeof(LeftBuild)), b_tile = al(ntiles * sizeof(TileDesc));
+    HIP_TRY(B.upload.reserve(b_pos + b_left + b_build + b_tile));
+    char* up = B.upload.as<char>();
+    std::memcpy(up, P.pos.data(), npos * sizeof(PosDesc));
+    std::memcpy(up + b_pos, P.lefts.data(), nleft * sizeof(LeftDesc));
+    std::memcpy(up + b_pos + b_left, P.builds.data(), nleft * sizeof(LeftBuild));
+    std::memcpy(up + b_pos + b_left + b_build, P.tiles.data(), ntiles * sizeof(TileDesc));
+
+    HIP_TRY(hipEventRecord(B.ev[0], st));
+    if (npos) HIP_TRY(hipMemcpyAsync(B.pos.p, up, npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
     if (nleft) {
-        HIP_TRY(hipMemcpyAsync(c->lefts.p, P.lefts.data(), nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(c->builds.p, P.builds.data(), nleft * sizeof(LeftBuild), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(B.lefts.p, up + b_pos, nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(B.builds.p, up + b_pos + b_left, nleft * sizeof(LeftBuild), hipMemcpyHostToDevice,
+                               st));
     }
     if (ntiles)
-        HIP_TRY(hipMemcpyAsync(c->tiles.p, P.tiles.data(), ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_build_left(c->builds.as<LeftBuild>(), c->lefts.as<LeftDesc>(), nleft, c->d_vbase, c->d_vre,
-                              c->d_vim, c->lre.as<i128>(), c->lim.as<i128>(), c->lfast.as<LeftFast>(), st));
+        HIP_TRY(hipMemcpyAsync(B.tiles.p, up + b_pos + b_left + b_build, ntiles * sizeof(TileDesc),
+                               hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_build_left(B.builds.as<LeftBuild>(), B.lefts.as<LeftDesc>(), nleft, c->d_vbase, c->d_vre,
+                              c->d_vim, B.lre.as<i128>(), B.lim.as<i128>(), B.lfast.as<LeftFast>(), st));
+    uint32_t* misc = B.misc.as<uint32_t>();  // [0] next tile, [1] deferred, [2] raw, [3] hits
+    HIP_TRY(launch_batch_reset(B.pos.as<PosDesc>(), npos, cnt.cand, cnt.leaves, cnt.pairs, misc, st));
     CellParams K{};
     K.off = c->d_off;
     K.cre = c->d_cre;
     K.cim = c->d_cim;
     K.cres = c->d_cres;
-    K.pos = c->pos.as<PosDesc>();
-    K.lefts = c->lefts.as<LeftDesc>();
-    K.tiles = c->tiles.as<TileDesc>();
+    K.pos = B.pos.as<PosDesc>();
+    K.lefts = B.lefts.as<LeftDesc>();
+    K.tiles = B.tiles.as<TileDesc>();
     K.ntiles = ntiles;
-    K.lfast = c->lfast.as<LeftFast>();
-    K.lre = c->lre.as<i128>();
-    K.lim = c->lim.as<i128>();
-    uint32_t* misc = c->misc.as<uint32_t>();  // [0] next tile, [1] deferred, [2] raw, [3] hits
+    K.lfast = B.lfast.as<LeftFast>();
+    K.lre = B.lre.as<i128>();
+    K.lim = B.lim.as<i128>();
     K.next_tile = misc;
-    K.cand = c->counters.as<unsigned long long>();
-    K.leaves = K.cand + npos;
-    K.pairs = K.cand + 2 * npos;
+    K.cand = cnt.cand;
+    K.leaves = cnt.leaves;
+    K.pairs = cnt.pairs;
+    K.def = B.def.as<DefItem>();
     K.ndef = misc + 1;
-    K.raw = c->raw.as<HitOut>();
+    K.def_cap = B.def_cap;
+    K.raw = B.raw.as<HitOut>();
     K.nraw = misc + 2;
-    K.raw_cap = raw_cap;
-    K.limit = limit;
-    double ld = (double)(limit * limit);
-    K.n4_d = ld * ld;
+    K.raw_cap = LaunchBuffers::RAW_CAP;
     K.qr = c->qr;
-    const uint32_t blocks = (uint32_t)(c->cus * c->blocks_per_cu);
-    c->stats.grid_blocks = blocks;
-    c->stats.block_threads = 256;
-    uint32_t ndef = 0;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        K.def = c->def.as<DefItem>();
-        K.def_cap = c->def_cap;
-        HIP_TRY(hipMemsetAsync(c->counters.p, 0, (size_t)npos * 3 * 8, st));
-        HIP_TRY(hipMemsetAsync(misc, 0, 16, st));
-        HIP_TRY(hipEventRecord(c->ev[1], st));
-        vlog(st, "left sets built");
-        if (ntiles) HIP_TRY(launch_cells(K, blocks, st));
-        vlog(st, "cell kernel");
-        HIP_TRY(hipEventRecord(c->ev[2], st));
-        HIP_TRY(hipMemcpyAsync(&ndef, misc + 1, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        if (ndef <= c->def_cap) break;
-        // deferred buffer overflowed: grow it and run the batch again
-        if (attempt == 3 || ndef > (1u << 30)) return fail(HSS_ENOMEM, "deferred-leaf buffer overflow");
-        c->def_cap = ndef + ndef / 2;
-        HIP_TRY(c->def.reserve((size_t)c->def_cap * sizeof(DefItem)));
-    }
-    HIP_TRY(launch_exact(K, ndef, st));
-    vlog(st, "exact kernel");
-    HIP_TRY(launch_accept(K.raw, K.nraw, raw_cap, limit, c->hits.as<HitOut>(), misc + 3, hit_cap, st));
-    HIP_TRY(hipEventRecord(c->ev[3], st));
-    cnt.resize((size_t)npos * 3);
-    HIP_TRY(hipMemcpyAsync(cnt.data(), c->counters.p, (size_t)npos * 3 * 8, hipMemcpyDeviceToHost, st));
-    uint32_t m4[4];
-    HIP_TRY(hipMemcpyAsync(m4, misc, 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (m4[2] > raw_cap) return fail(HSS_EOVERFLOW, "raw fourth-power buffer overflow");
-    if (m4[3] > hit_cap) return fail(HSS_EOVERFLOW, "accepted-hit buffer overflow");
-    raw_count = m4[2];
-    hits_out.resize(m4[3]);
-    if (m4[3])
-        HIP_TRY(hipMemcpy(hits_out.data(), c->hits.p, m4[3] * sizeof(HitOut), hipMemcpyDeviceToHost));
-    float a = 0, b = 0, d = 0;
-    (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
-    c->stats.setup_ms = a;
-    c->stats.cells_ms = b;
-    c->stats.exact_ms = d;
-    c->stats.tiles = ntiles;
-    c->stats.left_entries = P.left_entries;
-    c->stats.deferred = ndef;
-    c->stats.raw_roots = raw_count;
-    c->stats.total_ms = ms_since(t0);
+    HIP_TRY(hipEventRecord(B.ev[1], st));
+    vlog(st, "left sets built");
+    if (ntiles) HIP_TRY(launch_cells(K, c->grid_blocks(), st));
+    vlog(st, "cell kernel");
+    HIP_TRY(hipEventRecord(B.ev[2], st));
+    HIP_TRY(launch_exact(K, (uint32_t)c->cus * 4, st));
+    HIP_TRY(launch_accept(K.pos, K.raw, K.nraw, LaunchBuffers::RAW_CAP, B.hits.as<HitOut>(), misc + 3,
+                          LaunchBuffers::HIT_CAP, st));
+    vlog(st, "exact + accept");
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, misc, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(B.ev[3], st));
     return HSS_OK;
 }
+
+int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out) {
+    out = LaunchResult();
+    const uint32_t* m4 = B.h_misc.as<uint32_t>();
+    out.deferred = m4[1];
+    out.raw = m4[2];
+    (void)hipEventElapsedTime(&out.setup_ms, B.ev[0], B.ev[1]);
+    (void)hipEventElapsedTime(&out.cells_ms, B.ev[1], B.ev[2]);
+    (void)hipEventElapsedTime(&out.exact_ms, B.ev[2], B.ev[3]);
+    if (m4[1] > B.def_cap) {
+        // deferred buffer overflowed: the counts are incomplete; grow for a rerun
+        if (m4[1] > (1u << 30)) return fail(HSS_ENOMEM, "deferred-leaf buffer overflow");
+        B.def_cap = m4[1] + m4[1] / 2;
+        HIP_TRY(B.def.reserve((size_t)B.def_cap * sizeof(DefItem)));
+        out.rerun = true;
+        return HSS_OK;
+    }
+    if (m4[2] > LaunchBuffers::RAW_CAP) return fail(HSS_EOVERFLOW, "raw fourth-power buffer overflow");
+    if (m4[3] > LaunchBuffers::HIT_CAP) return fail(HSS_EOVERFLOW, "accepted-hit buffer overflow");
+    out.hits.resize(m4[3]);
+    if (m4[3]) {
+        HIP_TRY(hipMemcpyAsync(out.hits.data(), B.hits.p, m4[3] * sizeof(HitOut), hipMemcpyDeviceToHost, copy));
+        HIP_TRY(hipStreamSynchronize(copy));
+    }
+    (void)c;
+    return HSS_OK;
+}
+
+}  // namespace hss
 
 extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
                           hss_root_result* per_root, uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap,
                           uint32_t* n_hits) {
     if (!c) return fail(HSS_EINVAL, "null context");
+    if (c->server) return fail(HSS_EINVAL, "context is attached to a server");
     if (n_roots && !roots) return fail(HSS_EINVAL, "roots is null");
     if (hit_cap && !hits) return fail(HSS_EINVAL, "hits is null");
     if (hipSetDevice(c->device) != hipSuccess) return fail(HSS_EDEVICE, "hipSetDevice failed");
@@ -396,24 +398,53 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     if (n_hits) *n_hits = 0;
     if (candidates_total) *candidates_total = 0;
     if (!n_roots) return HSS_OK;
-    std::vector<unsigned long long> cnt;
-    std::vector<HitOut> raw_hits;
-    uint32_t raw_count = 0;
-    int rc = run_batch(c, limit, roots, n_roots, cnt, raw_hits, raw_count);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<PosReq> reqs(n_roots);
+    for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limit, s};
+    LaunchBuffers& B = c->sync;
+    int rc = launch_plan(c, B, reqs.data(), n_roots);
     if (rc != HSS_OK) return rc;
+    c->stats.plan_ms = ms_since(t0);
+    HIP_TRY(c->counters.reserve((size_t)n_roots * 3 * 8));
+    Counters cnt;
+    cnt.cand = c->counters.as<unsigned long long>();
+    cnt.leaves = cnt.cand + n_roots;
+    cnt.pairs = cnt.cand + 2 * (size_t)n_roots;
+    LaunchResult R;
+    for (int attempt = 0;; ++attempt) {
+        rc = launch_enqueue(c, B, cnt, c->stream);
+        if (rc != HSS_OK) return rc;
+        HIP_TRY(hipEventSynchronize(B.ev[3]));
+        rc = launch_collect(c, B, c->stream, R);
+        if (rc != HSS_OK) return rc;
+        if (!R.rerun) break;
+        if (attempt == 3) return fail(HSS_ENOMEM, "deferred-leaf buffer overflow");
+    }
+    std::vector<unsigned long long> h((size_t)n_roots * 3);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->counters.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     uint64_t total = 0, leaves = 0;
     for (uint32_t s = 0; s < n_roots; ++s) {
-        total += cnt[s];
-        leaves += cnt[n_roots + s];
-        if (per_root) per_root[s] = hss_root_result{cnt[s], cnt[n_roots + s], cnt[2 * n_roots + s]};
+        total += h[s];
+        leaves += h[n_roots + s];
+        if (per_root) per_root[s] = hss_root_result{h[s], h[n_roots + s], h[2 * (size_t)n_roots + s]};
     }
+    c->stats.setup_ms = R.setup_ms;
+    c->stats.cells_ms = R.cells_ms;
+    c->stats.exact_ms = R.exact_ms;
+    c->stats.tiles = B.plan.tiles.size();
+    c->stats.left_entries = B.plan.left_entries;
+    c->stats.deferred = R.deferred;
+    c->stats.raw_roots = R.raw;
+    c->stats.grid_blocks = c->grid_blocks();
+    c->stats.block_threads = 256;
     c->stats.candidates = total;
     c->stats.leaves = leaves;
     if (candidates_total) *candidates_total = total;
     std::set<std::tuple<uint64_t, uint64_t, uint64_t>> seen;
     std::vector<HitOut> uniq;
-    for (const HitOut& h : raw_hits)
-        if (seen.insert({h.l, h.m, h.n}).second) uniq.push_back(h);
+    for (const HitOut& x : R.hits)
+        if (seen.insert({x.l, x.m, x.n}).second) uniq.push_back(x);
     std::sort(uniq.begin(), uniq.end(), [](const HitOut& a, const HitOut& b) {
         return std::tie(a.l, a.m, a.n) < std::tie(b.l, b.m, b.n);
     });
@@ -421,6 +452,7 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     for (uint32_t i = 0; i < k && i < hit_cap; ++i)
         hits[i] = hss_hit{uniq[i].l, uniq[i].m, uniq[i].n, uniq[i].area4_lo, uniq[i].area4_hi};
     if (n_hits) *n_hits = k;
+    c->stats.total_ms = ms_since(t0);
     if (k > hit_cap) return fail(HSS_EOVERFLOW, "more hits than hit_cap");
     return HSS_OK;
 }

@@ -10,15 +10,18 @@ namespace hss {
 static constexpr int MAX_SPLIT = 8;  // distinct primes = 1 (mod 4) of an integer <= 2^26
 static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
 
-// One root position l of a batch (uniform across a tile).
+// One root position l of a batch (uniform across a tile).  Positions of one
+// batch may carry different LIMITs.
 struct PosDesc {
     uint64_t l;
-    double l4_d;  // FP64 value of l^4
-    double l2_d;  // l^2 (exact)
+    uint64_t limit;   // LIMIT of this position: partners and third sides <= limit
+    double n4_d;      // FP64 value of limit^4
+    double l4_d;      // FP64 value of l^4
+    double l2_d;      // l^2 (exact)
     uint32_t l3, l5;  // l mod 3, l mod 5
     uint32_t l_even;
     uint32_t nsplit, ninert;
-    uint32_t pad;
+    uint32_t out;     // index of this position's counters (candidates, leaves, pairs)
     // divisibility of a partner index k by the split / inert primes of l:
     // p | k  <=>  k * inv (mod 2^32) <= thr
     uint32_t split_inv[MAX_SPLIT], split_thr[MAX_SPLIT];

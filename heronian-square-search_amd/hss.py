@@ -14,7 +14,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libhss.so"
 
-HSS_OK, HSS_EINVAL, HSS_ENOMEM, HSS_EDEVICE, HSS_EOVERFLOW = 0, 1, 2, 3, 4
+HSS_OK, HSS_EINVAL, HSS_ENOMEM, HSS_EDEVICE, HSS_EOVERFLOW, HSS_EAGAIN = 0, 1, 2, 3, 4, 5
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class Hit(ctypes.Structure):
@@ -44,6 +45,36 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_slots", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("state_bytes", ctypes.c_uint64),
+                ("launch_work", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("limit", ctypes.c_uint64), ("first", ctypes.c_uint64),
+                ("last", ctypes.c_uint64), ("resume", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
+                ("leaves", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("n_hits", ctypes.c_uint32),
+                ("launches", ctypes.c_uint32), ("state", ctypes.c_int32), ("error", ctypes.c_int32),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("reruns", ctypes.c_uint64), ("shared_launches", ctypes.c_uint64),
+                ("positions", ctypes.c_uint64),
+                ("leaves", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("gpu_ms", ctypes.c_double),
+                ("uptime_ms", ctypes.c_double), ("queued", ctypes.c_uint32), ("active", ctypes.c_uint32),
+                ("max_active", ctypes.c_uint32), ("pool_positions", ctypes.c_uint32), ("pool_free", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     if not path.is_file():
         raise RuntimeError(f"{path} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
@@ -63,6 +94,16 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
                                         ctypes.POINTER(u32)]
     lib.hss_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.hss_verify_triple.argtypes = [u64, u64, u64, u64, u64]
+    lib.hss_estimate.argtypes = [vp, u64, u64, u64, ctypes.POINTER(ctypes.c_double)]
+    lib.hss_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.hss_server_submit.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64)]
+    lib.hss_server_wait.argtypes = [vp, u64, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
+    lib.hss_server_results.argtypes = [vp, u64, ctypes.POINTER(RootResult), u64, ctypes.POINTER(Hit), u32,
+                                       ctypes.POINTER(u32)]
+    lib.hss_server_release.argtypes = [vp, u64]
+    lib.hss_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    lib.hss_server_destroy.argtypes = [vp]
+    lib.hss_server_destroy.restype = None
     return lib
 
 
@@ -105,8 +146,11 @@ class Context:
         self._h = ctypes.c_void_p()
         _check(_lib.hss_ctx_create(n_max, device, ctypes.byref(self._h)))
         self.n_max = n_max
+        self._server = None
 
     def close(self):
+        if self._server is not None:
+            self._server.close()
         if self._h:
             _lib.hss_ctx_destroy(self._h)
             self._h = ctypes.c_void_p()
@@ -153,6 +197,99 @@ class Context:
         _check(_lib.hss_search_interval(self._h, limit, first, last, ctypes.byref(total), hits, hit_cap,
                                         ctypes.byref(nh)))
         return total.value, [(hits[i].l, hits[i].m, hits[i].n, hits[i].area) for i in range(nh.value)]
+
+    def estimate(self, limit: int, first: int, last: int) -> float:
+        """Planner work estimate (cells) of the positions [first, last] at limit."""
+        out = ctypes.c_double()
+        _check(_lib.hss_estimate(self._h, limit, first, last, ctypes.byref(out)))
+        return out.value
+
+
+@dataclass
+class ServedResult:
+    info: dict
+    candidates: list[int]
+    leaves: list[int]
+    pairs: list[int]
+    hits: list[tuple[int, int, int, int]]
+
+    @property
+    def total(self) -> int:
+        return sum(self.candidates)
+
+
+class Server:
+    """The served form over one Context: submit interval requests from any
+    thread, wait for them; requests share launches on the device (continuous
+    batching) and are admitted by request slots and device state memory.
+    The context cannot run Context.search while a server is attached."""
+
+    def __init__(self, ctx: Context, max_slots: int = 0, state_bytes: int = 0, launch_work: float = 0.0):
+        self._ctx = ctx
+        self._h = ctypes.c_void_p()
+        cfg = ServerConfig(max_slots, 0, state_bytes, launch_work)
+        _check(_lib.hss_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(self._h)))
+        ctx._server = self
+
+    def submit(self, limit: int, first: int, last: int) -> int:
+        rid = ctypes.c_uint64()
+        _check(_lib.hss_server_submit(self._h, limit, first, last, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> dict | None:
+        """Request info once done; None when still running at the timeout."""
+        info = RequestInfo()
+        rc = _lib.hss_server_wait(self._h, rid, timeout_ms, ctypes.byref(info))
+        if rc == HSS_EAGAIN:
+            return None
+        _check(rc)
+        return info.as_dict()
+
+    def info(self, rid: int) -> dict:
+        info = RequestInfo()
+        rc = _lib.hss_server_wait(self._h, rid, 0, ctypes.byref(info))
+        if rc not in (HSS_OK, HSS_EAGAIN):
+            _check(rc)
+        return info.as_dict()
+
+    def result(self, rid: int, hit_cap: int = 4096) -> ServedResult:
+        info = self.wait(rid)
+        n = info["last"] - info["first"] + 1
+        per = (RootResult * n)()
+        hits = (Hit * max(hit_cap, 1))()
+        nh = ctypes.c_uint32()
+        _check(_lib.hss_server_results(self._h, rid, per, n, hits, hit_cap, ctypes.byref(nh)))
+        return ServedResult(info, [per[i].candidates for i in range(n)], [per[i].leaves for i in range(n)],
+                            [per[i].pairs for i in range(n)],
+                            [(hits[i].l, hits[i].m, hits[i].n, hits[i].area) for i in range(nh.value)])
+
+    def release(self, rid: int):
+        _check(_lib.hss_server_release(self._h, rid))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(_lib.hss_server_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        """Finish every submitted request and detach from the context."""
+        if self._h:
+            _lib.hss_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            if self._ctx._server is self:
+                self._ctx._server = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def visible_device() -> int:

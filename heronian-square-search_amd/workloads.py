@@ -12,6 +12,10 @@ benchmark intervals:
 * ``uneven1536`` LIMIT 10^7: slice512 + 512 heavy roots + 512 light roots drawn
                  from [9*10^6, 10^7] with a fixed seed.
 
+The served stream (``served_stream``) is a seeded sequence of interval
+requests: LIMIT drawn from {10^5, 10^6, 10^7}, length log-uniform in
+[1, 4096], FIRST_L uniform in [1, LIMIT - length + 1], Poisson arrivals.
+
 Heavy roots have at least four distinct prime factors p = 1 (mod 4), so their
 Gaussian weight w(l) = prod(4e+1) over those primes is at least 625.  Light roots
 are prime powers with w in {1, 5, 9}.  Weights follow
@@ -94,3 +98,27 @@ def workload(name: str):
         heavy, light = uneven_roots()
         return 10_000_000, list(range(7_500_000, 7_500_512)) + heavy + light
     raise KeyError(name)
+
+
+STREAM_LIMITS = (100_000, 1_000_000, 10_000_000)
+
+
+def served_stream(n: int = 40, rate_hz: float = 5.0, seed: int = SEED):
+    """The first n requests of the seeded served stream.
+
+    Returns a list of (arrival_seconds, limit, first, last).  Request i draws
+    its LIMIT, length and FIRST_L in that order from one generator; arrival
+    gaps are exponential with mean 1/rate_hz, drawn from a second generator so
+    that the request set does not depend on the rate.  With the default seed
+    the first 40 requests hold about 1.2e12 planner cells of work.
+    """
+    rng = np.random.default_rng(seed)
+    gaps = np.random.default_rng(seed + 1)
+    out, t = [], 0.0
+    for _ in range(n):
+        limit = int(rng.choice(STREAM_LIMITS))
+        length = int(round(2 ** rng.uniform(0, 12)))
+        first = int(rng.integers(1, limit - length + 2))
+        t += float(gaps.exponential(1.0 / rate_hz))
+        out.append((t, limit, first, first + length - 1))
+    return out

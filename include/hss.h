@@ -30,6 +30,7 @@ extern "C" {
 #define HSS_ENOMEM 2    /* device or host allocation failed */
 #define HSS_EDEVICE 3   /* HIP runtime error or kernel failure */
 #define HSS_EOVERFLOW 4 /* more hits than the caller's buffer holds (count still reported) */
+#define HSS_EAGAIN 5    /* a served request is not finished (wait timed out) */
 
 typedef struct hss_ctx hss_ctx;
 
@@ -73,7 +74,8 @@ typedef struct {
  */
 int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out);
 
-/* Release a context (src/gpu_search_v42.cu:496-505). */
+/* Release a context (src/gpu_search_v42.cu:496-505); an attached server is
+ * destroyed first (hss_server_destroy). */
 void hss_ctx_destroy(hss_ctx* ctx);
 
 /* Thread-local description of the last failure in this thread. */
@@ -114,6 +116,96 @@ int hss_get_stats(const hss_ctx* ctx, hss_stats* out);
  * nondegenerate primitive triangle of that integer area, else 0.
  */
 int hss_verify_triple(uint64_t l, uint64_t m, uint64_t n, uint64_t area_lo, uint64_t area_hi);
+
+/*
+ * Work estimate of the root positions [first, last] at `limit`: the planner's
+ * count of (left entry, partner half-set entry) cells, the unit the server's
+ * launch budget and the tile sizes are measured in.  Host arithmetic only.
+ */
+int hss_estimate(const hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t last, double* cells);
+
+/* ------------------------------------------------------------ served form
+ *
+ * A server drives one context from its own thread.  Clients on any thread
+ * submit interval requests (LIMIT, FIRST_L, LAST_L) -- each is what one
+ * gpu_search_v42 process did for its interval (src/gpu_search_v42.cu:610-632)
+ * -- and wait for them.  Requests are admitted in arrival order while a
+ * request slot and room in the device state pool (24 bytes per root position:
+ * the request's per-position counters, resident on the device until it is
+ * done) are free; every launch takes work from all admitted requests, so
+ * requests that arrive while others run join the next launch and share the
+ * device.  Each position's counts and hits are the same as a search of that
+ * position alone.  The reference has no served form: its supervisor
+ * (scripts/gpu_queue.py:65-98) runs one process per interval and GPU.
+ */
+typedef struct hss_server hss_server;
+
+typedef struct {
+    uint32_t max_slots;   /* requests admitted at once; 0: 256 */
+    uint32_t reserved;
+    uint64_t state_bytes; /* device state pool, 24 bytes per root position; 0: 64 MiB */
+    double launch_work;   /* work per launch in planner cells (hss_estimate); 0: 3e10 */
+} hss_server_config;
+
+#define HSS_REQ_QUEUED 0 /* waiting for a slot and state memory */
+#define HSS_REQ_ACTIVE 1 /* admitted: positions are being searched */
+#define HSS_REQ_DONE 2
+#define HSS_REQ_FAILED 3
+
+typedef struct {
+    uint64_t id;
+    uint64_t limit, first, last;
+    uint64_t resume;     /* root sides [first, resume) are finished (a STATE file's resume) */
+    uint64_t candidates; /* totals, once done */
+    uint64_t leaves;
+    uint64_t pairs;
+    uint32_t n_hits;     /* once done */
+    uint32_t launches;   /* launches the request's positions were part of */
+    int32_t state;       /* HSS_REQ_* */
+    int32_t error;       /* status of a failed request */
+    /* server clock, milliseconds since hss_server_create; -1 until reached */
+    double submit_ms, admit_ms, start_ms, done_ms;
+} hss_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t launches, reruns;
+    uint64_t shared_launches; /* launches that carried positions of two or more requests */
+    uint64_t positions; /* root positions of completed requests */
+    uint64_t leaves, candidates;
+    double gpu_ms;      /* sum of launch durations (HIP events) */
+    double uptime_ms;
+    uint32_t queued, active, max_active;
+    uint32_t pool_positions, pool_free;
+    uint32_t reserved;
+} hss_server_stats;
+
+/* Attach a server to ctx (one per context; hss_search on ctx is refused while
+ * it is attached). */
+int hss_server_create(hss_ctx* ctx, const hss_server_config* cfg, hss_server** out);
+
+/* Queue the request [first, last] at limit (first >= 1, last <= limit <= the
+ * context's n_max, at most the pool's positions); *id receives its handle. */
+int hss_server_submit(hss_server* srv, uint64_t limit, uint64_t first, uint64_t last, uint64_t* id);
+
+/* Wait up to timeout_ms (< 0: no limit) for a request; fills *info (nullable)
+ * either way.  HSS_OK when done, HSS_EAGAIN when still running, the request's
+ * error when it failed. */
+int hss_server_wait(hss_server* srv, uint64_t id, int64_t timeout_ms, hss_request_info* info);
+
+/* Results of a done request: per-position counters (cap entries, last-first+1
+ * needed) and its hits (de-duplicated, sorted); HSS_EOVERFLOW when a buffer
+ * is short (counts still reported). */
+int hss_server_results(hss_server* srv, uint64_t id, hss_root_result* per_root, uint64_t cap,
+                       hss_hit* hits, uint32_t hit_cap, uint32_t* n_hits);
+
+/* Forget a finished request. */
+int hss_server_release(hss_server* srv, uint64_t id);
+
+int hss_server_get_stats(hss_server* srv, hss_server_stats* out);
+
+/* Finish every submitted request, stop the server thread and detach. */
+void hss_server_destroy(hss_server* srv);
 
 #ifdef __cplusplus
 }

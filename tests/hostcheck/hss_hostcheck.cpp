@@ -18,6 +18,7 @@
 
 #include "hss_build.h"
 #include "hss_plan.h"
+#include "hss_sched.h"
 
 using namespace hss;
 
@@ -119,15 +120,19 @@ typedef struct {
     uint64_t cells;
 } hc_stats;
 
-// Search roots at limit with the kernels' arithmetic.  Tables are rebuilt when
-// nmax changes.  Returns 0 on success.
-int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint64_t* cand,
-              uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits, hc_stats* st) {
+// Search root positions (roots[s] at limits[s]) with the kernels' arithmetic.
+// Counters are indexed by position.  Tables are rebuilt when nmax changes.
+// Returns 0 on success.
+int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits, uint32_t n_roots, uint64_t* cand,
+                    uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits,
+                    hc_stats* st) {
     if (g_tables.n != nmax) build(g_tables, nmax);
     const HostTables& T = g_tables;
     Plan P;
     std::string err;
-    if (!plan_batch(limit, roots, n_roots, T.small, T.off.data(), nmax, 4096, P, err)) return 1;
+    std::vector<PosReq> reqs(n_roots);
+    for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limits[s], s};
+    if (!plan_batch(reqs.data(), n_roots, T.small, T.off.data(), nmax, 4096, P, err)) return 1;
     // left sets
     std::vector<i128> lre(P.left_entries), lim(P.left_entries);
     std::vector<LeftFast> lfast(P.left_entries);
@@ -157,7 +162,6 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
         uint32_t left, ui, vidx, k, conj, full;
     };
     std::vector<Def> defs;
-    double ld = (double)(limit * limit), n4 = ld * ld;
     for (const TileDesc& Tl : P.tiles) {
         const LeftDesc& L = P.lefts[Tl.left];
         const PosDesc& D = P.pos[L.pos];
@@ -168,16 +172,16 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
             if (!ok) continue;
             uint32_t Hk = T.off[k + 1] - T.off[k];
             if (L.trivial && Hk == 1) continue;
-            pairs[L.pos] += 1;
+            pairs[D.out] += 1;
             uint64_t m = (uint64_t)L.q * k;
             LaneCtx ctx;
-            make_lane_ctx(m, D.l4_d, D.l2_d, n4, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
+            make_lane_ctx(m, D.l4_d, D.l2_d, D.n4_d, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
             for (uint32_t jj = 0; jj < Hk; ++jj) {
                 uint32_t vidx = T.off[k] + jj;
                 bool vc = jj + 1 == Hk;
                 double c = T.cre[vidx], d = T.cim[vidx];
                 uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
-                leaves[L.pos] += vc ? L.H : 2ull * L.H - 1;
+                leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
                 for (uint32_t ui = 0; ui < L.H; ++ui) {
                     const LeftFast& u = lfast[L.off + ui];
                     bool uc = ui + 1 == L.H;
@@ -211,7 +215,7 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
                             ++S.deferred_full;
                             continue;
                         }
-                        cand[L.pos] += r.cnt;
+                        cand[D.out] += r.cnt;
                         for (int tm = 0; tm < 4; ++tm)
                             if (r.f[tm] && root_near_integer(r.zz[tm])) {
                                 defs.push_back({Tl.left, ui, vidx, k, cj, 0});
@@ -231,16 +235,16 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
         const PosDesc& D = P.pos[L.pos];
         uint64_t m = (uint64_t)L.q * df.k;
         ExactCtx e;
-        make_exact_ctx(D.l, m, limit, e);
+        make_exact_ctx(D.l, m, D.limit, e);
         i128 zre, zim;
         exact_product(lre[L.off + df.ui], lim[L.off + df.ui], (int64_t)T.cre[df.vidx], (int64_t)T.cim[df.vidx],
                       df.conj != 0, zre, zim);
         uint32_t c = exact_leaf(zre, zim, e, qr, [&](uint64_t r, u128 a4) {
             ++S.raw_roots;
-            if (accept_hit(D.l, m, r, limit, a4) && seen.insert({D.l, m, r}).second)
+            if (accept_hit(D.l, m, r, D.limit, a4) && seen.insert({D.l, m, r}).second)
                 got.push_back(hc_hit{D.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64)});
         });
-        if (df.full) cand[L.pos] += c;
+        if (df.full) cand[D.out] += c;
     }
     std::sort(got.begin(), got.end(),
               [](const hc_hit& a, const hc_hit& b) { return std::tie(a.l, a.m, a.n) < std::tie(b.l, b.m, b.n); });
@@ -248,6 +252,19 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
     *n_hits = (uint32_t)got.size();
     if (st) *st = S;
     return 0;
+}
+
+// All positions at one limit.
+int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint64_t* cand,
+              uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits, hc_stats* st) {
+    std::vector<uint64_t> limits(n_roots, limit);
+    return hc_search_mixed(nmax, roots, limits.data(), n_roots, cand, leaves, pairs, hits, hit_cap, n_hits, st);
+}
+
+// Planner work estimate of one position (cells), or -1.
+double hc_estimate(uint64_t nmax, uint64_t l, uint64_t limit) {
+    if (g_tables.n != nmax) build(g_tables, nmax);
+    return estimate_position(l, limit, g_tables.small, g_tables.off.data(), nmax);
 }
 
 // Cache entry e of the host tables (for cross-checks of the table layout).
@@ -262,5 +279,58 @@ int hc_cache_entry(uint64_t nmax, uint32_t i, uint32_t t, double* re, double* im
     *r180 = T.cres[T.off[i] + t];
     return 0;
 }
+
+// ---- request scheduler of the served form (hss_sched.h), driven from tests
+
+void* hc_sched_new(uint32_t max_slots, uint64_t pool_positions, double launch_work, uint32_t max_launch_positions) {
+    SchedConfig c;
+    c.max_slots = max_slots;
+    c.pool_positions = pool_positions;
+    c.launch_work = launch_work;
+    c.max_launch_positions = max_launch_positions;
+    return new Sched(c);
+}
+
+void hc_sched_free(void* h) { delete static_cast<Sched*>(h); }
+
+int hc_sched_enqueue(void* h, uint64_t id, uint32_t npos) { return static_cast<Sched*>(h)->enqueue(id, npos) ? 1 : 0; }
+
+// admitted ids into out[0..cap); returns their number
+uint32_t hc_sched_admit(void* h, uint64_t* out, uint32_t cap) {
+    std::vector<uint64_t> a = static_cast<Sched*>(h)->admit();
+    for (uint32_t i = 0; i < a.size() && i < cap; ++i) out[i] = a[i];
+    return (uint32_t)a.size();
+}
+
+void hc_sched_set_costs(void* h, uint64_t id, const double* cost, uint32_t n) {
+    static_cast<Sched*>(h)->set_costs(id, std::vector<double>(cost, cost + n));
+}
+
+// segments of the next launch; returns their number
+uint32_t hc_sched_next(void* h, uint64_t* ids, uint32_t* begin, uint32_t* end, uint32_t cap) {
+    std::vector<Segment> s = static_cast<Sched*>(h)->next_launch();
+    for (uint32_t i = 0; i < s.size() && i < cap; ++i) {
+        ids[i] = s[i].id;
+        begin[i] = s[i].begin;
+        end[i] = s[i].end;
+    }
+    return (uint32_t)s.size();
+}
+
+uint32_t hc_sched_finished(void* h, const uint64_t* ids, const uint32_t* begin, const uint32_t* end, uint32_t n,
+                           uint64_t* out, uint32_t cap) {
+    std::vector<Segment> s(n);
+    for (uint32_t i = 0; i < n; ++i) s[i] = Segment{ids[i], begin[i], end[i]};
+    std::vector<uint64_t> d = static_cast<Sched*>(h)->finished(s);
+    for (uint32_t i = 0; i < d.size() && i < cap; ++i) out[i] = d[i];
+    return (uint32_t)d.size();
+}
+
+void hc_sched_retire(void* h, uint64_t id) { static_cast<Sched*>(h)->retire(id); }
+uint32_t hc_sched_pool_offset(void* h, uint64_t id) { return static_cast<Sched*>(h)->pool_offset(id); }
+uint32_t hc_sched_prefix(void* h, uint64_t id) { return static_cast<Sched*>(h)->prefix(id); }
+uint64_t hc_sched_pool_free(void* h) { return static_cast<Sched*>(h)->pool_free(); }
+uint32_t hc_sched_queued(void* h) { return (uint32_t)static_cast<Sched*>(h)->queued(); }
+uint32_t hc_sched_active(void* h) { return (uint32_t)static_cast<Sched*>(h)->active(); }
 
 }  // extern "C"

@@ -46,6 +46,31 @@ def test_struct_layouts():
         assert re.search(rf"\b{field}\b", text), field
 
 
+def test_struct_layouts_match_c(tmp_path):
+    """sizeof and every field offset of the ABI structs, compiled from the header
+    by the host C compiler, against the ctypes binding."""
+    import hss
+    structs = {"hss_hit": hss.Hit, "hss_root_result": hss.RootResult, "hss_stats": hss.Stats,
+               "hss_server_config": hss.ServerConfig, "hss_request_info": hss.RequestInfo,
+               "hss_server_stats": hss.ServerStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
+
 def test_verify_triple_known_answers():
     import hss
     assert hss.verify_triple(1853, 4380, 4427, 32918611718880)

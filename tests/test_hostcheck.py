@@ -94,3 +94,42 @@ def test_interval_limit_1e5(hc, golden):
     g = golden["i100000_99901_100000"]
     c = hc_search(hc, 100000, 100000, list(range(g["first"], g["last"] + 1)))[0]
     assert sum(c) == g["candidates"]
+
+
+def hc_search_mixed(hc, nmax, positions):
+    n = len(positions)
+    R = (ctypes.c_uint64 * n)(*[l for l, _ in positions])
+    L = (ctypes.c_uint64 * n)(*[lim for _, lim in positions])
+    c, lv, pr = (ctypes.c_uint64 * n)(), (ctypes.c_uint64 * n)(), (ctypes.c_uint64 * n)()
+    H = (HcHit * 64)()
+    nh = ctypes.c_uint32()
+    rc = hc.hc_search_mixed(ctypes.c_uint64(nmax), R, L, n, c, lv, pr, H, 64, ctypes.byref(nh), None)
+    assert rc == 0
+    hits = [(h.l, h.m, h.n, ((h.hi << 64) | h.lo) // 4) for h in H[: nh.value]]
+    return list(c), list(lv), list(pr), hits
+
+
+def test_mixed_limits_in_one_batch(hc, golden):
+    """Positions at different LIMITs planned into one batch count exactly as
+    each LIMIT alone (the served form mixes requests of several LIMITs)."""
+    positions = [(1853, 5000), (11789, 70000), (1853, 70000), (4999, 5000), (30000, 70000), (1, 5000)]
+    c, lv, pr, hits = hc_search_mixed(hc, 70000, positions)
+    for i, (l, lim) in enumerate(positions):
+        a = hc_search(hc, 70000, lim, [l])
+        assert (c[i], lv[i], pr[i]) == (a[0][0], a[1][0], a[2][0]), (l, lim)
+    assert c[1] == golden["known2"]["candidates"]
+    # the hit of l=1853 is found at LIMIT 5000 (n = 4427) and again at 70000
+    assert hits.count((1853, 4380, 4427, 32918611718880)) == 1
+    assert (11789, 68104, 68595, 284239560530875680) in hits
+
+
+def test_estimate_bounds_leaves(hc):
+    """The planner's work estimate (cells) bounds the enumerated leaves:
+    every cell yields at most two leaves (u*v and u*conj(v))."""
+    for l, lim in [(1, 5000), (1853, 5000), (11789, 70000), (4225, 70000), (65537, 70000)]:
+        est = hc.hc_estimate
+        est.restype = ctypes.c_double
+        e = est(ctypes.c_uint64(70000), ctypes.c_uint64(l), ctypes.c_uint64(lim))
+        lv = hc_search(hc, 70000, lim, [l])[1][0]
+        assert 0 < lv <= 2 * e, (l, lim, lv, e)
+    assert hc.hc_estimate(ctypes.c_uint64(70000), ctypes.c_uint64(5001), ctypes.c_uint64(5000)) == -1.0
