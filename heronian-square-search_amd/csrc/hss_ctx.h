@@ -60,7 +60,7 @@ struct Counters {
 // Everything one launch of the search needs besides the resident tables.  A
 // buffer set is reused only after its launch has completed.
 struct LaunchBuffers {
-    DevBuf pos, lefts, builds, tiles, lfast, lre, lim, misc, def, raw, hits;
+    DevBuf pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
     HostBuf upload;   // pinned staging of the plan
     HostBuf h_misc;   // pinned: [0] tile head, [1] deferred, [2] raw, [3] hits
     uint32_t def_cap = 0;
@@ -108,10 +108,13 @@ struct hss_ctx {
     hss::LaunchBuffers sync;
     hss::DevBuf counters;
     hss_stats stats{};
+    hss::Tuning tune{};
+    int win_blocks_per_cu = 1;
     hss_server* server = nullptr;  // a server drives the context while attached
 
     uint64_t waves() const { return (uint64_t)cus * blocks_per_cu * 4; }
     uint32_t grid_blocks() const { return (uint32_t)(cus * blocks_per_cu); }
+    uint32_t win_grid_blocks() const { return (uint32_t)(cus * win_blocks_per_cu); }
 };
 
 namespace hss {

@@ -172,7 +172,7 @@ __global__ void k_prime_values(const uint32_t* plist, uint32_t np, const uint32_
 }
 
 // one cache entry per thread
-__global__ void k_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
+__global__ __launch_bounds__(256) void k_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
                              const uint32_t* vbase, const i128* vre, const i128* vim, double* cre, double* cim,
                              uint32_t* cres) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,20 +220,31 @@ __global__ void k_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t 
     cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
 }
 
-// one block per left set
-__global__ void k_build_left(const LeftBuild* builds, const LeftDesc* lefts, const uint32_t* vbase,
-                             const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast) {
+// Gaussian value of rank t of a left set: fixed * prod of the merged prime
+// values selected by the digits of t (decode_rank order)
+__device__ inline GInt left_value(const LeftBuild& b, uint32_t t, const uint32_t* vbase, const i128* vre,
+                                  const i128* vim) {
+    uint32_t dig[MAX_SPLIT];
+    decode_rank(t, b.k, b.E, dig);
+    GInt z = {(i128)b.fixed, 0};
+    for (uint32_t j = 0; j < b.k; ++j) {
+        uint32_t at = vbase[b.p[j]] + exp_offset(b.E[j]) + dig[j];
+        z = gmul(z, GInt{vre[at], vim[at]});
+    }
+    return z;
+}
+
+// One block per left set.  Entry t0 holds rank perm[off + t0] for a sorted
+// left set (window mode), rank t0 otherwise.
+__global__ __launch_bounds__(256) void k_build_left(const LeftBuild* builds, const LeftDesc* lefts, const uint32_t* vbase,
+                             const i128* vre, const i128* vim, const uint32_t* perm, i128* lre, i128* lim,
+                             LeftFast* lfast) {
     const LeftBuild b = builds[blockIdx.x];
     const LeftDesc L = lefts[blockIdx.x];
     for (uint32_t t0 = threadIdx.x; t0 < L.H; t0 += blockDim.x) {
-        uint32_t dig[MAX_SPLIT];
-        decode_rank(t0, b.k, b.E, dig);
-        GInt z = {(i128)b.fixed, 0};
-        for (uint32_t j = 0; j < b.k; ++j) {
-            uint32_t at = vbase[b.p[j]] + exp_offset(b.E[j]) + dig[j];
-            z = gmul(z, GInt{vre[at], vim[at]});
-        }
-        uint32_t e = L.off + t0;
+        const uint32_t e = L.off + t0;
+        const uint32_t rank = L.bt != NO_TABLE ? perm[e] : t0;
+        GInt z = left_value(b, rank, vbase, vre, vim);
         lre[e] = z.re;
         lim[e] = z.im;
         LeftFast f;
@@ -242,6 +253,68 @@ __global__ void k_build_left(const LeftBuild* builds, const LeftDesc* lefts, con
         f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
         f.pad = 0;
         lfast[e] = f;
+    }
+}
+
+// One block per sorted left set (index i of the plan's sorted list): order
+// the non-central entries by psi (angle modulo pi/2, from the same FP64
+// coordinates k_build_left stores), write the permutation and the bucket
+// table bt[b] = number of entries with psi_bucket < b, b = 0..WIN_NB.
+__global__ __launch_bounds__(256) void k_left_sort(const uint32_t* sorted, const LeftBuild* builds,
+                                                   const LeftDesc* lefts, const uint32_t* vbase, const i128* vre,
+                                                   const i128* vim, uint32_t* perm, uint32_t* btab) {
+    __shared__ double key[WIN_MAX_SORT];
+    __shared__ uint16_t idx[WIN_MAX_SORT];
+    const uint32_t li = sorted[blockIdx.x];
+    const LeftBuild b = builds[li];
+    const LeftDesc L = lefts[li];
+    const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
+    uint32_t n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) {
+        double k = 1e300;
+        if (t < n) {
+            GInt z = left_value(b, t, vbase, vre, vim);
+            k = psi_of(i128_to_double(z.re), i128_to_double(z.im));
+        }
+        key[t] = k;
+        idx[t] = (uint16_t)t;
+    }
+    __syncthreads();
+    // bitonic sort of (key, idx) over n2 entries
+    for (uint32_t size = 2; size <= n2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+                const uint32_t i = 2 * t - (t & (stride - 1));
+                const uint32_t j = i + stride;
+                const bool up = (i & size) == 0;
+                const double ki = key[i], kj = key[j];
+                const bool sw = up ? (ki > kj || (ki == kj && idx[i] > idx[j]))
+                                   : (ki < kj || (ki == kj && idx[i] < idx[j]));
+                if (sw) {
+                    key[i] = kj;
+                    key[j] = ki;
+                    const uint16_t ti = idx[i];
+                    idx[i] = idx[j];
+                    idx[j] = ti;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) perm[L.off + t] = idx[t];
+    if (threadIdx.x == 0) perm[L.off + n] = n;  // the central entry stays last
+    uint32_t* bt = btab + (size_t)L.bt * (WIN_NB + 1);
+    for (uint32_t bk = threadIdx.x; bk <= WIN_NB; bk += blockDim.x) {
+        uint32_t lo = 0, hi = n;  // first t with psi_bucket(key[t]) >= bk
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (psi_bucket<WIN_NB>(key[mid]) < bk)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bt[bk] = bk == WIN_NB ? n : lo;
     }
 }
 
@@ -455,11 +528,207 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
     }
 }
 
+// ------------------------------------------------------ window-mode kernel
+//
+// Tiles whose left set is stored sorted by psi (hss_math.h, range windows).
+// Per chunk of 64 partner entries (one per lane, mapped as in cell_kernel):
+//   phase 1 (lane = partner entry v): the central u leaf as in cell_kernel;
+//     the index ranges of the left entries whose leaf u*v or u*conj(v) can
+//     count; the lane's partner data, fast-path context and ranges go to a
+//     per-wave LDS record;
+//   phase 2 (lane = one (v, range entry) item): the wave walks the flattened
+//     items of all 64 partners 64 at a time, each lane evaluating one leaf
+//     with leaf_fast from its owner's record and a vector load of u.
+// Leaves outside the ranges count 0 exactly and are skipped.
+
+struct WinRec {
+    double c, d;  // partner v
+    double t1_lo_w, t1_lo_n, t1_hi_w, t1_hi_n, t2_w, t2_n, tau, lm;
+    uint32_t inv_h, thr_h;
+    uint32_t cres;   // (c mod 180) | (d mod 180) << 16
+    uint32_t vidx;
+    uint32_t kj;
+    uint32_t pad;
+    uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
+    uint16_t cum[8];  // items before each range
+};
+
+__global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
+    __shared__ double s_qz[WAVES][QCAP];
+    __shared__ uint4 s_qid[WAVES][QCAP];
+    __shared__ WinRec s_rec[WAVES][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    RootQueue rq{s_qz[wave], s_qid[wave], 0};
+    WinRec* rec = s_rec[wave];
+    for (;;) {
+        const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_wtile, lane == 0 ? 1u : 0u));
+        if (t >= P.nwtiles) break;
+        const TileDesc T = P.wtiles[t];
+        const LeftDesc L = P.lefts[T.left];
+        const PosDesc& S = P.pos[L.pos];
+        const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
+        const uint32_t nsort = L.H - 1;
+        uint64_t cand = 0, leaves = 0, pairs = 0;
+        for (uint32_t par = 0; par < 2; ++par) {
+            uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
+            for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
+                const uint32_t k = kb + 2u * (uint32_t)lane;
+                const uint32_t kq = k <= T.k1 ? k : T.k1;
+                bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
+                for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
+                for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
+                uint32_t Hk = P.off[kq + 1] - P.off[kq];
+                Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
+                pairs += Hk ? 1 : 0;
+                uint32_t incl = wave_incl_scan(Hk, lane);
+                uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+                for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                    // ---- phase 1: lane = partner entry
+                    uint32_t e = e0 + (uint32_t)lane;
+                    bool act = e < total;
+                    uint32_t j = 0;
+                    for (uint32_t s = 32; s > 0; s >>= 1) {
+                        uint32_t v = __shfl(incl, (int)(j + s - 1), 64);
+                        if (v <= e) j += s;
+                    }
+                    j = j > 63 ? 63 : j;
+                    uint32_t inclj = __shfl(incl, (int)j, 64);
+                    uint32_t Hj = __shfl(Hk, (int)j, 64);
+                    const uint32_t kj = kb + 2u * j;
+                    const uint32_t jj = e - (inclj - Hj);
+                    const uint32_t kjq = act ? kj : T.k0;
+                    const uint32_t vidx = act ? P.off[kjq] + jj : 0u;
+                    const bool vc = jj + 1 == Hj;
+                    const double c = P.cre[vidx], d = P.cim[vidx];
+                    const uint32_t vr = P.cres[vidx];
+                    const uint32_t c180 = vr & 0xFFFFu, d180 = vr >> 16;
+                    const uint64_t m = (uint64_t)L.q * kj;
+                    const uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
+                    LaneCtx ctx;
+                    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, ctx);
+                    leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
+                    {  // the central (real) u, as in cell_kernel
+                        const uint32_t ui = L.H - 1;
+                        const LeftFast u = P.lfast[L.off + ui];
+                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                        const double t1 = u.im * d, t2 = u.re * d;
+                        const uint32_t vX = ua * c180 + QA * QA - ub * d180, vY = ua * d180 + ub * c180;
+                        LeafEval rr;
+                        eval_real_leaf(vX, ctx, rr);
+                        const bool real = act && vc;
+                        cand += (real && !rr.und) ? rr.cnt : 0u;
+                        push_global(P, real && rr.und, DefItem{T.left, ui, vidx, kj | 0x80000000u});
+                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
+                                  vX, vY, ctx, ui, vidx, kj, cand);
+                    }
+                    // ranges of non-central entries
+                    const double m2 = (double)(m * m);
+                    Window w;
+                    make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, ctx.tau, 2.0 * S.l2_d * m2, w);
+                    const double pv = psi_of_f(c, d);
+                    uint32_t rs[8], re[8];
+                    uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
+                    uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
+                    n0 = act ? n0 : 0u;
+                    n1 = (act && !vc) ? n1 : 0u;
+                    const uint32_t cnt = n0 + n1;
+                    __builtin_amdgcn_wave_barrier();
+                    WinRec& R = rec[lane];
+                    R.c = c;
+                    R.d = d;
+                    R.t1_lo_w = ctx.t1_lo_w;
+                    R.t1_lo_n = ctx.t1_lo_n;
+                    R.t1_hi_w = ctx.t1_hi_w;
+                    R.t1_hi_n = ctx.t1_hi_n;
+                    R.t2_w = ctx.t2_w;
+                    R.t2_n = ctx.t2_n;
+                    R.tau = ctx.tau;
+                    R.lm = ctx.lm;
+                    R.inv_h = ctx.inv_h;
+                    R.thr_h = ctx.thr_h;
+                    R.cres = vr;
+                    R.vidx = vidx;
+                    R.kj = kj;
+                    R.pad = 0;
+                    uint32_t acc = 0;
+                    for (int p = 0; p < 8; ++p) {
+                        const bool live = p < 4 ? n0 != 0 : n1 != 0;
+                        const uint32_t len = live ? re[p] - rs[p] : 0u;
+                        R.st[p] = (uint16_t)rs[p];
+                        R.cum[p] = (uint16_t)acc;
+                        acc += len;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // ---- phase 2: lane = item
+                    const uint32_t icl = wave_incl_scan(cnt, lane);
+                    const uint32_t items = __builtin_amdgcn_readfirstlane(__shfl(icl, 63, 64));
+                    for (uint32_t i0 = 0; i0 < items; i0 += 64) {
+                        const uint32_t it = i0 + (uint32_t)lane;
+                        const bool live = it < items;
+                        uint32_t o = 0;
+                        for (uint32_t s = 32; s > 0; s >>= 1) {
+                            uint32_t v = __shfl(icl, (int)(o + s - 1), 64);
+                            if (v <= it) o += s;
+                        }
+                        o = o > 63 ? 63 : o;
+                        const uint32_t iclo = __shfl(icl, (int)o, 64);
+                        const uint32_t cnto = __shfl(cnt, (int)o, 64);
+                        const uint32_t q = live ? it - (iclo - cnto) : 0u;
+                        const WinRec& Q = rec[o];
+                        uint32_t pc = 0;
+                        for (int p = 1; p < 8; ++p) pc += Q.cum[p] <= q ? 1u : 0u;
+                        const uint32_t ui0 = (uint32_t)Q.st[pc] + (q - (uint32_t)Q.cum[pc]);
+                        const uint32_t ui = live && ui0 < nsort ? ui0 : 0u;  // in-bounds for every lane
+                        const bool cj = pc >= 4;
+                        const LeftFast u = P.lfast[L.off + ui];
+                        const double qc = Q.c, qd = Q.d;
+                        const uint32_t qr = Q.cres;
+                        const uint32_t qc180 = qr & 0xFFFFu, qd180 = qr >> 16;
+                        LaneCtx qx;
+                        qx.t1_lo_w = Q.t1_lo_w;
+                        qx.t1_lo_n = Q.t1_lo_n;
+                        qx.t1_hi_w = Q.t1_hi_w;
+                        qx.t1_hi_n = Q.t1_hi_n;
+                        qx.t2_w = Q.t2_w;
+                        qx.t2_n = Q.t2_n;
+                        qx.tau = Q.tau;
+                        qx.lm = Q.lm;
+                        qx.inv_h = Q.inv_h;
+                        qx.thr_h = Q.thr_h;
+                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                        const double t1 = u.im * qd, t2 = u.re * qd;
+                        const uint32_t pac = ua * qc180, pbd = ub * qd180, pad = ua * qd180, pbc = ub * qc180;
+                        const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
+                        const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
+                        const uint32_t vX = cj ? pac + pbd : pac + QA * QA - pbd;
+                        const uint32_t vY = cj ? pbc + QA * QA - pad : pad + pbc;
+                        leaf_fast(P, rq, T.left, live, X, Y, vX, vY, qx, ui, Q.vidx, Q.kj | (cj ? 1u << 30 : 0u),
+                                  cand);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        if (rq.n) {
+            __builtin_amdgcn_wave_barrier();
+            root_drain(P, rq, T.left, rq.n);
+            rq.n = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        cand = wave_sum64(cand);
+        leaves = wave_sum64(leaves);
+        pairs = wave_sum64(pairs);
+        atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
+        atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
+        atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
+    }
+}
+
 // Exact settlement of deferred leaves: full reference chain with 128-bit
 // integers; counts are added only for leaves the fast path left undecided.
 // The item count is read on the device (an overflowed buffer holds def_cap
 // items; the host then runs the batch again with a larger buffer).
-__global__ void k_exact(const CellParams P) {
+__global__ __launch_bounds__(256) void k_exact(const CellParams P) {
     const uint32_t n = *P.ndef < P.def_cap ? *P.ndef : P.def_cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const DefItem it = P.def[i];
@@ -484,7 +753,7 @@ __global__ void k_exact(const CellParams P) {
 // Acceptance of the raw exact fourth powers: gcd, 5 | lmn, ordering and the
 // 256-bit Heron recheck against the position's own LIMIT; accepted triples
 // are appended to the hit list.
-__global__ void k_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
+__global__ __launch_bounds__(256) void k_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
                          uint32_t* nhits, uint32_t hit_cap) {
     uint32_t n = *nraw < raw_cap ? *nraw : raw_cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -496,13 +765,13 @@ __global__ void k_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* 
     }
 }
 
-// Start of a batch: zero the work-queue head and the deferred / raw / hit
-// counters (misc[0..3]) and the counters of the batch's positions.  A kernel
+// Start of a batch: zero the work-queue heads and the deferred / raw / hit
+// counters (misc[0..7]) and the counters of the batch's positions.  A kernel
 // rather than memset nodes, so the launch sequence can be captured in a graph.
 __global__ void k_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand,
                               unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 4) misc[i] = 0;
+    if (i < 8) misc[i] = 0;
     for (; i < npos; i += gridDim.x * blockDim.x) {
         const uint32_t o = pos[i].out;
         cand[o] = 0;
@@ -566,15 +835,24 @@ hipError_t launch_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t 
 }
 
 hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uint32_t nleft, const uint32_t* vbase,
-                             const i128* vre, const i128* vim, i128* lre, i128* lim, LeftFast* lfast,
-                             hipStream_t st) {
+                             const i128* vre, const i128* vim, const uint32_t* sorted, uint32_t nsorted,
+                             uint32_t* perm, uint32_t* btab, i128* lre, i128* lim, LeftFast* lfast, hipStream_t st) {
     if (!nleft) return hipSuccess;
-    hipLaunchKernelGGL(k_build_left, dim3(nleft), dim3(256), 0, st, builds, lefts, vbase, vre, vim, lre, lim, lfast);
+    if (nsorted)
+        hipLaunchKernelGGL(k_left_sort, dim3(nsorted), dim3(256), 0, st, sorted, builds, lefts, vbase, vre, vim, perm,
+                           btab);
+    hipLaunchKernelGGL(k_build_left, dim3(nleft), dim3(256), 0, st, builds, lefts, vbase, vre, vim,
+                       (const uint32_t*)perm, lre, lim, lfast);
     return hipGetLastError();
 }
 
 hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st) {
     hipLaunchKernelGGL(cell_kernel, dim3(blocks), dim3(WG), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_win(const CellParams& P, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(cell_win_kernel, dim3(blocks), dim3(WG), 0, st, P);
     return hipGetLastError();
 }
 
@@ -599,6 +877,10 @@ hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long l
 
 hipError_t cell_occupancy(int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, cell_kernel, WG, 0);
+}
+
+hipError_t cell_win_occupancy(int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, cell_win_kernel, WG, 0);
 }
 
 }  // namespace hss

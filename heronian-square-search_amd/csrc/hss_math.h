@@ -29,6 +29,7 @@
 // integer fourth root and the acceptance rules are evaluated.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -218,6 +219,208 @@ HSS_HD inline void eval_real_leaf(uint32_t vX, const LaneCtx& c, LeafEval& r) {
     r.und = a2 && c.t2_w >= 0.0 && c.t2_n < 0.0;
     r.f[0] = r.f[1] = r.f[2] = r.f[3] = false;
     r.zz[0] = r.zz[1] = r.zz[2] = r.zz[3] = 0.0;
+}
+
+// ----------------------------------------------------------- range windows
+//
+// A leaf z = 2uv has |z| = R = 2 l^2 m^2, so its range terms depend only on
+// psi = arg z modulo pi/2 (the four terms are invariant under z -> iz and
+// z -> conj z).  With S = [s0, s1] the set of |D| / R that can pass or be
+// undecided (range_terms widened by tau), a term can count only when
+// sin psi or cos psi lies in S:
+//     psi in A = [asin s0, asin s1]  or  psi in A' = pi/2 - A.
+// Left sets in window mode are sorted by psi_u; since psi(uv) = psi_u + psi_v
+// and psi(u conj v) = psi_u - psi_v (mod pi/2), the entries u that can give a
+// counted leaf with a partner v form a few contiguous index ranges, found
+// through a bucket table.  Every other leaf counts 0 exactly, so skipping it
+// changes no count.
+//
+// Error budget of the window.  s0, s1 carry FP64 errors < 1e-15 and are
+// widened by 1e-6 before rounding to FP32 (rounding error < 6e-8), so the
+// rounded arguments still enclose [s0, s1] and asinf is evaluated on an
+// enclosing interval; asinf (<= 4 ulp, < 5e-7 rad) and the partner angle from
+// atan2f on FP32 coordinates (coordinate rounding < 1.2e-7 rad, atan2f <= 3 ulp
+// < 3.6e-7 rad) add < 1e-6 rad together; the sort keys psi_u are FP64.  The
+// angle window is widened by WIN_EPS = 2e-6 rad.  FP32 keeps the register
+// footprint of the hot kernel low; the bounds only have to enclose.
+
+static constexpr double HALF_PI = 1.5707963267948966;
+static constexpr double WIN_EPS = 2e-6;
+
+// Angle of (x, y) != (0, 0) modulo pi/2, in [0, pi/2), FP64 (sort keys).
+HSS_HD inline double psi_of(double x, double y) {
+    double a, b;  // rotate by a power of i into a > 0, b >= 0
+    if (x > 0 && y >= 0) {
+        a = x;
+        b = y;
+    } else if (x <= 0 && y > 0) {
+        a = y;
+        b = -x;
+    } else if (x < 0 && y <= 0) {
+        a = -x;
+        b = -y;
+    } else {
+        a = -y;
+        b = x;
+    }
+    double r = atan2(b, a);
+    return r < HALF_PI ? r : 0.0;  // an angle that rounds to pi/2 is 0 modulo pi/2
+}
+
+// The same angle from FP32 coordinates and atan2f (error < 5e-7 rad).
+HSS_HD inline double psi_of_f(double x, double y) {
+    float a, b;
+    if (x > 0 && y >= 0) {
+        a = (float)x;
+        b = (float)y;
+    } else if (x <= 0 && y > 0) {
+        a = (float)y;
+        b = (float)-x;
+    } else if (x < 0 && y <= 0) {
+        a = (float)-x;
+        b = (float)-y;
+    } else {
+        a = (float)-y;
+        b = (float)x;
+    }
+    double r = (double)atan2f(b, a);
+    return r < HALF_PI ? r : 0.0;
+}
+
+// Bucket of an angle in [0, pi/2] (WIN_NB buckets).
+template <uint32_t NB>
+HSS_HD inline uint32_t psi_bucket(double t) {
+    double b = t * ((double)NB / HALF_PI);
+    uint32_t k = b > 0 ? (uint32_t)b : 0u;
+    return k < NB ? k : NB - 1;
+}
+
+// psi window of the pair (l, m): n = 1 or 2 intervals [a_i, b_i], or n = -1
+// for the whole circle.  lo, hi, tau as in make_lane_ctx; R = 2 l^2 m^2.
+struct Window {
+    double a0, b0, a1, b1;
+    int n;
+};
+
+HSS_HD inline void make_window(double lo, double hi, double tau, double R, Window& w) {
+    double s0, s1;
+    if (lo <= tau) {
+        s0 = 0.0;
+        s1 = ((hi > -lo ? hi : -lo) + tau) / R;
+    } else {
+        s0 = (lo - tau) / R;
+        s1 = (hi + tau) / R;
+    }
+    s0 -= 1e-6;
+    s1 += 1e-6;
+    s0 = s0 < 0.0 ? 0.0 : (s0 > 1.0 ? 1.0 : s0);
+    s1 = s1 < 0.0 ? 0.0 : (s1 > 1.0 ? 1.0 : s1);
+    const double a0 = (double)asinf((float)s0) - WIN_EPS, a1 = (double)asinf((float)s1) + WIN_EPS;
+    const double q = 0.5 * HALF_PI;
+    if (a0 <= q && a1 >= q) {  // A and A' overlap: one interval
+        w.a0 = a0 < HALF_PI - a1 ? a0 : HALF_PI - a1;
+        w.b0 = a1 > HALF_PI - a0 ? a1 : HALF_PI - a0;
+        w.a1 = w.b1 = 0.0;
+        w.n = (w.b0 - w.a0 >= HALF_PI) ? -1 : 1;
+    } else {
+        w.a0 = a0;
+        w.b0 = a1;
+        w.a1 = HALF_PI - a1;
+        w.b1 = HALF_PI - a0;
+        w.n = 2;
+    }
+}
+
+// Fraction of the psi circle a window covers (planning: work of window mode).
+HSS_HD inline double window_fraction(const Window& w) {
+    if (w.n < 0) return 1.0;
+    double f = (w.b0 - w.a0) + (w.n == 2 ? w.b1 - w.a1 : 0.0);
+    f /= HALF_PI;
+    return f > 1.0 ? 1.0 : f;
+}
+
+// Index range [s, e) of the sorted entries whose bucket lies in
+// [bucket(x), bucket(y)], 0 <= x <= y <= pi/2.  bt[b] = number of entries with
+// bucket < b (bt[NB] = all sorted entries).
+template <uint32_t NB>
+HSS_HD inline void bucket_range(const uint32_t* bt, double x, double y, uint32_t& s, uint32_t& e) {
+    s = bt[psi_bucket<NB>(x)];
+    e = bt[psi_bucket<NB>(y) + 1];
+}
+
+// Ranges of entries u with psi_u in the cyclic interval [a + shift, b + shift]
+// (b - a < pi/2): one or two index ranges; the second is empty when unused.
+template <uint32_t NB>
+HSS_HD inline void cyclic_ranges(const uint32_t* bt, double a, double b, double shift, uint32_t& s0, uint32_t& e0,
+                                 uint32_t& s1, uint32_t& e1) {
+    double x = a + shift;
+    x -= HALF_PI * floor(x / HALF_PI);
+    x = x < 0.0 ? 0.0 : (x >= HALF_PI ? 0.0 : x);
+    const double y = x + (b - a);
+    if (y < HALF_PI) {
+        bucket_range<NB>(bt, x, y, s0, e0);
+        s1 = e1 = 0;
+    } else {
+        bucket_range<NB>(bt, x, HALF_PI, s0, e0);
+        bucket_range<NB>(bt, 0.0, y - HALF_PI, s1, e1);
+    }
+}
+
+// Sort four ranges by start and merge overlapping ones in place; merged-away
+// and empty ranges end as [0, 0).  Returns the number of entries covered.
+HSS_HD inline uint32_t merge4(uint32_t* s, uint32_t* e) {
+    for (int i = 0; i < 4; ++i)
+        if (s[i] >= e[i]) s[i] = e[i] = 0xFFFFFFFFu;  // empties sort last
+    auto cs = [&](int i, int j) {
+        const bool sw = s[j] < s[i];
+        const uint32_t ts = sw ? s[j] : s[i], te = sw ? e[j] : e[i];
+        s[j] = sw ? s[i] : s[j];
+        e[j] = sw ? e[i] : e[j];
+        s[i] = ts;
+        e[i] = te;
+    };
+    cs(0, 1);
+    cs(2, 3);
+    cs(0, 2);
+    cs(1, 3);
+    cs(1, 2);
+    for (int i = 1; i < 4; ++i)
+        for (int j = 0; j < i; ++j) {
+            const bool live = s[j] != 0xFFFFFFFFu && s[i] != 0xFFFFFFFFu;
+            const bool ov = live && s[i] <= e[j];
+            e[j] = ov ? (e[i] > e[j] ? e[i] : e[j]) : e[j];
+            s[i] = ov ? 0xFFFFFFFFu : s[i];
+            e[i] = ov ? 0xFFFFFFFFu : e[i];
+        }
+    uint32_t n = 0;
+    for (int i = 0; i < 4; ++i) {
+        const bool empty = s[i] == 0xFFFFFFFFu;
+        n += empty ? 0u : e[i] - s[i];
+        s[i] = empty ? 0u : s[i];
+        e[i] = empty ? 0u : e[i];
+    }
+    return n;
+}
+
+// Entry ranges (up to four, merged) of a left set in window mode for the
+// leaves u*v (conj = false: psi_u in W - psi_v) or u*conj(v) (W + psi_v).
+// nsort = entries in sorted order (the central entry is excluded).
+template <uint32_t NB>
+HSS_HD inline uint32_t window_ranges(const Window& w, const uint32_t* bt, uint32_t nsort, double psi_v, bool conj,
+                                     uint32_t* s, uint32_t* e) {
+    const double shift = conj ? psi_v : -psi_v;
+    if (w.n < 0) {
+        s[0] = 0;
+        e[0] = nsort;
+        s[1] = e[1] = s[2] = e[2] = s[3] = e[3] = 0;
+        return nsort;
+    }
+    cyclic_ranges<NB>(bt, w.a0, w.b0, shift, s[0], e[0], s[1], e[1]);
+    if (w.n == 2)
+        cyclic_ranges<NB>(bt, w.a1, w.b1, shift, s[2], e[2], s[3], e[3]);
+    else
+        s[2] = e[2] = s[3] = e[3] = 0;
+    return merge4(s, e);
 }
 
 // ------------------------------------------------------------ exact stage

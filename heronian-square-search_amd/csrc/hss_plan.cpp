@@ -116,8 +116,24 @@ double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>
     return est;
 }
 
+double tile_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1, int samples) {
+    const double l2 = (double)(l * l), l4 = l2 * l2;
+    const double nd = (double)(limit * limit), n4 = nd * nd;
+    double sum = 0;
+    for (int i = 0; i < samples; ++i) {
+        const uint64_t k = samples == 1 ? k0 : k0 + (uint64_t)(k1 - k0) * i / (samples - 1);
+        const uint64_t m = q * k;
+        const double m2 = (double)(m * m), m4 = m2 * m2;
+        const double lo = (l4 + m4) - n4, tau = (4.0 * (l2 * m2) + n4) * 0x1p-44;
+        Window w;
+        make_window(lo, l4, tau, 2.0 * l2 * m2, w);
+        sum += window_fraction(w);
+    }
+    return sum / samples;
+}
+
 bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t>& small_primes, const uint32_t* off,
-                uint64_t n_max, uint64_t waves, Plan& P, std::string& err) {
+                uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& P, std::string& err) {
     P = Plan();
     P.pos.resize(n_roots);
     for (uint32_t s = 0; s < n_roots; ++s) {
@@ -188,18 +204,41 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
         err = "batch too large: split it";
         return false;
     }
+    // Window mode per left set: eligible by size, then per tile by the
+    // window fraction of its partner range.  Work is estimated in plain-leaf
+    // units: a plain cell costs 1, a window-mode cell f * win_item_cost plus
+    // win_partner_cost / H per partner entry.
+    const uint32_t nl = (uint32_t)P.lefts.size();
+    std::vector<uint8_t> eligible(nl, 0);
+    std::vector<double> gain(nl, 1.0);  // cost per cell of the whole left set
+    for (uint32_t li = 0; li < nl; ++li) {
+        const LeftDesc& L = P.lefts[li];
+        const uint32_t H = L.H;
+        if (H < tune.win_min_h || H < WIN_MIN_SORT + 1 || H > WIN_MAX_SORT + 1) continue;
+        eligible[li] = 1;
+        const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, P.kmin[li], P.kmax[li], 9);
+        if (f <= tune.win_max_frac) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
+        if (gain[li] > 1.0) gain[li] = 1.0;
+    }
+    double total = 0;
+    for (uint32_t li = 0; li < nl; ++li)
+        total += (double)P.lefts[li].H * (double)entries(off, P.kmin[li], P.kmax[li]) * gain[li];
     // tile size: a few tiles per resident wavefront, bounded
-    double target = P.est_cells / (double)(waves ? waves * 8 : 8);
+    const uint32_t tpw = tune.tiles_per_wave ? tune.tiles_per_wave : 8;
+    double target = total / (double)(waves ? waves * tpw : tpw);
     target = std::min(std::max(target, 8192.0), 4194304.0);
     struct Cost {
         double c;
         uint32_t i;
+        bool win;
     };
     std::vector<Cost> order;
-    for (uint32_t li = 0; li < P.lefts.size(); ++li) {
+    std::vector<TileDesc> all;
+    std::vector<uint8_t> sort_me(nl, 0);
+    for (uint32_t li = 0; li < nl; ++li) {
         const LeftDesc& L = P.lefts[li];
         uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
-        uint64_t budget = (uint64_t)std::max(target / (double)L.H, 256.0);
+        uint64_t budget = (uint64_t)std::max(target / ((double)L.H * gain[li]), 256.0);
         while (k0 <= kmax) {
             // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
             uint32_t lo = k0, hi = kmax;
@@ -211,18 +250,36 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                     hi = mid - 1;
             }
             uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
-            TileDesc T{li, k0, kend, 0};
-            order.push_back({(double)L.H * (double)entries(off, k0, kend), (uint32_t)P.tiles.size()});
-            P.tiles.push_back(T);
+            const double cells = (double)L.H * (double)entries(off, k0, kend);
+            double cost = cells;
+            bool win = false;
+            if (eligible[li]) {
+                const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);
+                if (f <= tune.win_max_frac) {
+                    win = true;
+                    cost = cells * f * tune.win_item_cost +
+                           (double)entries(off, k0, kend) * tune.win_partner_cost;
+                    sort_me[li] = 1;
+                    P.win_cells += cells;
+                }
+            }
+            P.est_work += cost;
+            order.push_back({cost, (uint32_t)all.size(), win});
+            all.push_back(TileDesc{li, k0, kend, 0});
             if (kend == kmax) break;
             k0 = kend + 1;
         }
     }
-    // heaviest tiles first
+    for (uint32_t li = 0; li < nl; ++li) {
+        P.lefts[li].bt = NO_TABLE;
+        if (sort_me[li]) {
+            P.lefts[li].bt = (uint32_t)P.sorted.size();
+            P.sorted.push_back(li);
+        }
+    }
+    // heaviest tiles first, per mode
     std::sort(order.begin(), order.end(), [](const Cost& a, const Cost& b) { return a.c > b.c; });
-    std::vector<TileDesc> sorted(P.tiles.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = P.tiles[order[i].i];
-    P.tiles.swap(sorted);
+    for (const Cost& c : order) (c.win ? P.wtiles : P.tiles).push_back(all[c.i]);
     return true;
 }
 

@@ -21,14 +21,31 @@
 
 namespace hss {
 
+// Scheduling knobs of one batch.  A tile runs in window mode (cell_win_kernel)
+// when its left set has win_min_h <= H <= WIN_MAX_SORT + 1 entries and its
+// mean range-window fraction is at most win_max_frac; otherwise in plain mode
+// (cell_kernel).  Keyed on H, i.e. on the Gaussian weight of l and of the
+// shared prime powers: H = (w(l; E) + 1) / 2.
+struct Tuning {
+    uint32_t win_min_h = 64;
+    double win_max_frac = 0.5;
+    double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf
+    double win_partner_cost = 4;  // per partner entry, in plain-leaf units
+    uint32_t tiles_per_wave = 8;  // tile size: est. work / (waves * tiles_per_wave)
+};
+
 struct Plan {
     std::vector<PosDesc> pos;
     std::vector<LeftDesc> lefts;
     std::vector<LeftBuild> builds;
-    std::vector<TileDesc> tiles;
+    std::vector<TileDesc> tiles;    // plain mode, heaviest first
+    std::vector<TileDesc> wtiles;   // window mode, heaviest first
+    std::vector<uint32_t> sorted;   // left sets stored sorted; LeftDesc::bt = index here
     std::vector<uint32_t> kmin, kmax;  // partner range per left set
     uint64_t left_entries = 0;
     double est_cells = 0;   // upper bound of (left entry, partner entry) cells
+    double est_work = 0;    // estimated work after window restriction (plain-leaf units / 2)
+    double win_cells = 0;   // cells of the window-mode tiles
 };
 
 // primes up to and including `upto`
@@ -49,7 +66,11 @@ struct PosReq {
 // Positions may repeat (the same l twice, or at two LIMITs): each is its own
 // unit of work.
 bool plan_batch(const PosReq* reqs, uint32_t n, const std::vector<uint32_t>& small_primes, const uint32_t* off,
-                uint64_t n_max, uint64_t waves, Plan& out, std::string& err);
+                uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& out, std::string& err);
+
+// Mean range-window fraction over partners m = q*k, k in [k0, k1], of root l at
+// limit (sampled at a few k).
+double tile_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1, int samples);
 
 // Work estimate of one root position in cells (left entry x partner half-set
 // entry), the same measure plan_batch sizes tiles with; -1 if l cannot be

@@ -73,7 +73,9 @@ hipError_t LaunchBuffers::create() {
 }
 
 void LaunchBuffers::release() {
-    for (DevBuf* b : {&pos, &lefts, &builds, &tiles, &lfast, &lre, &lim, &misc, &def, &raw, &hits}) b->release();
+    for (DevBuf* b : {&pos, &lefts, &builds, &tiles, &wtiles, &sorted, &perm, &btab, &lfast, &lre, &lim, &misc, &def,
+                      &raw, &hits})
+        b->release();
     upload.release();
     h_misc.release();
     for (hipEvent_t& e : ev) {
@@ -253,11 +255,38 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
     int bpc = 0;
     if (cell_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
     c->blocks_per_cu = bpc;
+    bpc = 0;
+    if (cell_win_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
+    c->win_blocks_per_cu = bpc;
     vlog(nullptr, "stream + occupancy");
     int rc = build_tables(c);
     vlog(c->stream, "tables built");
     if (rc != HSS_OK) return bail(rc);
     *out = c;
+    return HSS_OK;
+}
+
+extern "C" int hss_ctx_get_tuning(const hss_ctx* c, hss_tuning* out) {
+    if (!c || !out) return fail(HSS_EINVAL, "null argument");
+    out->win_min_h = c->tune.win_min_h;
+    out->tiles_per_wave = c->tune.tiles_per_wave;
+    out->win_max_frac = c->tune.win_max_frac;
+    out->win_item_cost = c->tune.win_item_cost;
+    out->win_partner_cost = c->tune.win_partner_cost;
+    return HSS_OK;
+}
+
+extern "C" int hss_ctx_set_tuning(hss_ctx* c, const hss_tuning* in) {
+    if (!c || !in) return fail(HSS_EINVAL, "null argument");
+    if (c->server) return fail(HSS_EINVAL, "context is attached to a server");
+    if (!in->tiles_per_wave || in->tiles_per_wave > 1024) return fail(HSS_EINVAL, "tiles_per_wave outside [1, 1024]");
+    if (!(in->win_max_frac >= 0.0 && in->win_max_frac <= 1.0)) return fail(HSS_EINVAL, "win_max_frac outside [0, 1]");
+    if (!(in->win_item_cost > 0.0) || !(in->win_partner_cost >= 0.0)) return fail(HSS_EINVAL, "bad cost weights");
+    c->tune.win_min_h = in->win_min_h;
+    c->tune.tiles_per_wave = in->tiles_per_wave;
+    c->tune.win_max_frac = in->win_max_frac;
+    c->tune.win_item_cost = in->win_item_cost;
+    c->tune.win_partner_cost = in->win_partner_cost;
     return HSS_OK;
 }
 
@@ -271,7 +300,7 @@ namespace hss {
 
 int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
     std::string err;
-    if (!plan_batch(reqs, n, c->small_primes, c->h_off.data(), c->nmax, c->waves(), B.plan, err))
+    if (!plan_batch(reqs, n, c->small_primes, c->h_off.data(), c->nmax, c->waves(), c->tune, B.plan, err))
         return fail(HSS_EINVAL, err);
     B.npos = n;
     return HSS_OK;
@@ -280,11 +309,16 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
 int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
     const Plan& P = B.plan;
     const uint32_t npos = B.npos, nleft = (uint32_t)P.lefts.size(), ntiles = (uint32_t)P.tiles.size();
+    const uint32_t nwtiles = (uint32_t)P.wtiles.size(), nsorted = (uint32_t)P.sorted.size();
     const size_t entries = std::max<uint64_t>(P.left_entries, 1);
     HIP_TRY(B.pos.reserve(std::max<size_t>(npos, 1) * sizeof(PosDesc)));
     HIP_TRY(B.lefts.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftDesc)));
     HIP_TRY(B.builds.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftBuild)));
     HIP_TRY(B.tiles.reserve(std::max<size_t>(ntiles, 1) * sizeof(TileDesc)));
+    HIP_TRY(B.wtiles.reserve(std::max<size_t>(nwtiles, 1) * sizeof(TileDesc)));
+    HIP_TRY(B.sorted.reserve(std::max<size_t>(nsorted, 1) * 4));
+    HIP_TRY(B.btab.reserve(std::max<size_t>(nsorted, 1) * (WIN_NB + 1) * 4));
+    HIP_TRY(B.perm.reserve(entries * 4));
     HIP_TRY(B.lfast.reserve(entries * sizeof(LeftFast)));
     HIP_TRY(B.lre.reserve(entries * sizeof(i128)));
     HIP_TRY(B.lim.reserve(entries * sizeof(i128)));
@@ -298,27 +332,39 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     // stage the plan in pinned memory: one region per array, 256-byte aligned
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t b_pos = al(npos * sizeof(PosDesc)), b_left = al(nleft * sizeof(LeftDesc)),
-                 b_build = al(nleft * sizeof(LeftBuild)), b_tile = al(ntiles * sizeof(TileDesc));
-    HIP_TRY(B.upload.reserve(b_pos + b_left + b_build + b_tile));
+                 b_build = al(nleft * sizeof(LeftBuild)), b_tile = al(ntiles * sizeof(TileDesc)),
+                 b_wtile = al(nwtiles * sizeof(TileDesc)), b_sorted = al(nsorted * 4);
+    HIP_TRY(B.upload.reserve(b_pos + b_left + b_build + b_tile + b_wtile + b_sorted));
     char* up = B.upload.as<char>();
-    std::memcpy(up, P.pos.data(), npos * sizeof(PosDesc));
-    std::memcpy(up + b_pos, P.lefts.data(), nleft * sizeof(LeftDesc));
-    std::memcpy(up + b_pos + b_left, P.builds.data(), nleft * sizeof(LeftBuild));
-    std::memcpy(up + b_pos + b_left + b_build, P.tiles.data(), ntiles * sizeof(TileDesc));
+    char* const up_pos = up;
+    char* const up_left = up_pos + b_pos;
+    char* const up_build = up_left + b_left;
+    char* const up_tile = up_build + b_build;
+    char* const up_wtile = up_tile + b_tile;
+    char* const up_sorted = up_wtile + b_wtile;
+    std::memcpy(up_pos, P.pos.data(), npos * sizeof(PosDesc));
+    std::memcpy(up_left, P.lefts.data(), nleft * sizeof(LeftDesc));
+    std::memcpy(up_build, P.builds.data(), nleft * sizeof(LeftBuild));
+    std::memcpy(up_tile, P.tiles.data(), ntiles * sizeof(TileDesc));
+    std::memcpy(up_wtile, P.wtiles.data(), nwtiles * sizeof(TileDesc));
+    std::memcpy(up_sorted, P.sorted.data(), nsorted * 4);
 
     HIP_TRY(hipEventRecord(B.ev[0], st));
-    if (npos) HIP_TRY(hipMemcpyAsync(B.pos.p, up, npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
+    if (npos) HIP_TRY(hipMemcpyAsync(B.pos.p, up_pos, npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
     if (nleft) {
-        HIP_TRY(hipMemcpyAsync(B.lefts.p, up + b_pos, nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(B.builds.p, up + b_pos + b_left, nleft * sizeof(LeftBuild), hipMemcpyHostToDevice,
-                               st));
+        HIP_TRY(hipMemcpyAsync(B.lefts.p, up_left, nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(B.builds.p, up_build, nleft * sizeof(LeftBuild), hipMemcpyHostToDevice, st));
     }
-    if (ntiles)
-        HIP_TRY(hipMemcpyAsync(B.tiles.p, up + b_pos + b_left + b_build, ntiles * sizeof(TileDesc),
-                               hipMemcpyHostToDevice, st));
+    if (ntiles) HIP_TRY(hipMemcpyAsync(B.tiles.p, up_tile, ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    if (nwtiles)
+        HIP_TRY(hipMemcpyAsync(B.wtiles.p, up_wtile, nwtiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    if (nsorted) HIP_TRY(hipMemcpyAsync(B.sorted.p, up_sorted, nsorted * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_build_left(B.builds.as<LeftBuild>(), B.lefts.as<LeftDesc>(), nleft, c->d_vbase, c->d_vre,
-                              c->d_vim, B.lre.as<i128>(), B.lim.as<i128>(), B.lfast.as<LeftFast>(), st));
-    uint32_t* misc = B.misc.as<uint32_t>();  // [0] next tile, [1] deferred, [2] raw, [3] hits
+                              c->d_vim, B.sorted.as<uint32_t>(), nsorted, B.perm.as<uint32_t>(),
+                              B.btab.as<uint32_t>(), B.lre.as<i128>(), B.lim.as<i128>(), B.lfast.as<LeftFast>(),
+                              st));
+    // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile
+    uint32_t* misc = B.misc.as<uint32_t>();
     HIP_TRY(launch_batch_reset(B.pos.as<PosDesc>(), npos, cnt.cand, cnt.leaves, cnt.pairs, misc, st));
     CellParams K{};
     K.off = c->d_off;
@@ -329,10 +375,14 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.lefts = B.lefts.as<LeftDesc>();
     K.tiles = B.tiles.as<TileDesc>();
     K.ntiles = ntiles;
+    K.wtiles = B.wtiles.as<TileDesc>();
+    K.nwtiles = nwtiles;
+    K.btab = B.btab.as<uint32_t>();
     K.lfast = B.lfast.as<LeftFast>();
     K.lre = B.lre.as<i128>();
     K.lim = B.lim.as<i128>();
     K.next_tile = misc;
+    K.next_wtile = misc + 4;
     K.cand = cnt.cand;
     K.leaves = cnt.leaves;
     K.pairs = cnt.pairs;
@@ -345,6 +395,7 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.qr = c->qr;
     HIP_TRY(hipEventRecord(B.ev[1], st));
     vlog(st, "left sets built");
+    if (nwtiles) HIP_TRY(launch_cells_win(K, c->win_grid_blocks(), st));
     if (ntiles) HIP_TRY(launch_cells(K, c->grid_blocks(), st));
     vlog(st, "cell kernel");
     HIP_TRY(hipEventRecord(B.ev[2], st));
@@ -432,7 +483,9 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     c->stats.setup_ms = R.setup_ms;
     c->stats.cells_ms = R.cells_ms;
     c->stats.exact_ms = R.exact_ms;
-    c->stats.tiles = B.plan.tiles.size();
+    c->stats.tiles = B.plan.tiles.size() + B.plan.wtiles.size();
+    c->stats.window_tiles = B.plan.wtiles.size();
+    c->stats.sorted_lefts = B.plan.sorted.size();
     c->stats.left_entries = B.plan.left_entries;
     c->stats.deferred = R.deferred;
     c->stats.raw_roots = R.raw;

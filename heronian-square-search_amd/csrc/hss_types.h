@@ -9,6 +9,13 @@ namespace hss {
 
 static constexpr int MAX_SPLIT = 8;  // distinct primes = 1 (mod 4) of an integer <= 2^26
 static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
+// Window mode: left sets with WIN_MIN_SORT <= H <= WIN_MAX_SORT + 1 are stored
+// sorted by the angle of their entries modulo pi/2 (the central entry stays
+// last) with a table of WIN_NB + 1 bucket starts.
+static constexpr uint32_t WIN_NB = 1024;
+static constexpr uint32_t WIN_MAX_SORT = 4096;
+static constexpr uint32_t WIN_MIN_SORT = 8;
+static constexpr uint32_t NO_TABLE = 0xFFFFFFFFu;
 
 // One root position l of a batch (uniform across a tile).  Positions of one
 // batch may carry different LIMITs.
@@ -37,7 +44,7 @@ struct LeftDesc {
     uint32_t q;        // product of the shared prime powers: m = q * k
     uint32_t trivial;  // l has no split prime: partners without one are skipped
     uint32_t q3, q5;   // q mod 3, q mod 5
-    uint32_t pad;
+    uint32_t bt;       // bucket table index of a sorted left set, else NO_TABLE
 };
 
 // Build recipe of one left set: fixed part and merged exponents of l's split primes.

@@ -39,10 +39,22 @@ class Stats(ctypes.Structure):
                 ("exact_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("tiles", ctypes.c_uint64),
                 ("left_entries", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("raw_roots", ctypes.c_uint64),
                 ("leaves", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
-                ("block_threads", ctypes.c_uint32)]
+                ("block_threads", ctypes.c_uint32), ("window_tiles", ctypes.c_uint64),
+                ("sorted_lefts", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("win_min_h", ctypes.c_uint32), ("tiles_per_wave", ctypes.c_uint32), ("win_max_frac", ctypes.c_double),
+                ("win_item_cost", ctypes.c_double), ("win_partner_cost", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+WINDOW_OFF = 0xFFFFFFFF
 
 
 class ServerConfig(ctypes.Structure):
@@ -93,6 +105,8 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.hss_search_interval.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(Hit), u32,
                                         ctypes.POINTER(u32)]
     lib.hss_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.hss_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.hss_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     lib.hss_verify_triple.argtypes = [u64, u64, u64, u64, u64]
     lib.hss_estimate.argtypes = [vp, u64, u64, u64, ctypes.POINTER(ctypes.c_double)]
     lib.hss_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -175,6 +189,19 @@ class Context:
         s = Stats()
         _check(_lib.hss_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(_lib.hss_ctx_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw) -> dict:
+        """Change scheduling knobs (counts never depend on them); returns the old ones."""
+        old = self.get_tuning()
+        new = dict(old, **kw)
+        t = Tuning(**new)
+        _check(_lib.hss_ctx_set_tuning(self._h, ctypes.byref(t)))
+        return old
 
     def search(self, limit: int, roots, hit_cap: int = 4096) -> SearchResult:
         roots = [int(r) for r in roots]

@@ -63,6 +63,8 @@ typedef struct {
     uint64_t candidates;
     uint32_t grid_blocks;
     uint32_t block_threads;
+    uint64_t window_tiles; /* tiles run by the window-mode kernel (range windows) */
+    uint64_t sorted_lefts; /* left sets stored sorted by angle for it */
 } hss_stats;
 
 /*
@@ -105,6 +107,27 @@ int hss_search(hss_ctx* ctx, uint64_t limit, const uint64_t* roots, uint32_t n_r
 int hss_search_interval(hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t last,
                         uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap,
                         uint32_t* n_hits);
+
+/*
+ * Scheduling knobs (no counterpart in the reference, whose kernel has one
+ * fixed launch shape, src/gpu_search_v42.cu:457-458).  A tile -- one left set
+ * of a root against a range of partners -- runs in window mode (only the left
+ * entries whose leaves can pass the range test, found from angle-sorted left
+ * sets) when the left set has at least win_min_h half-set entries (H =
+ * (w + 1) / 2 for its Gaussian weight w) and its mean range-window fraction is
+ * at most win_max_frac; otherwise every leaf is evaluated.  Counts never
+ * depend on these values; only speed does.
+ */
+typedef struct {
+    uint32_t win_min_h;      /* 0xFFFFFFFF: window mode off */
+    uint32_t tiles_per_wave; /* tile size: estimated work / (resident waves * this) */
+    double win_max_frac;
+    double win_item_cost;    /* planning cost of a window-mode leaf vs a plain leaf */
+    double win_partner_cost; /* planning cost per partner entry of a window-mode tile */
+} hss_tuning;
+
+int hss_ctx_get_tuning(const hss_ctx* ctx, hss_tuning* out);
+int hss_ctx_set_tuning(hss_ctx* ctx, const hss_tuning* in);
 
 /* Timings and work counters of the last hss_search / hss_search_interval call. */
 int hss_get_stats(const hss_ctx* ctx, hss_stats* out);

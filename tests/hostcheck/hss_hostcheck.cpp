@@ -117,12 +117,24 @@ typedef struct {
     uint64_t deferred_full;  // leaves left undecided by the FP64 band
     uint64_t deferred_root;  // counted terms near an exact fourth root
     uint64_t raw_roots;
-    uint64_t cells;
+    uint64_t cells;          // leaves evaluated
+    uint64_t window_tiles;
+    uint64_t window_items;   // leaves evaluated in window-mode tiles
 } hc_stats;
 
-// Search root positions (roots[s] at limits[s]) with the kernels' arithmetic.
-// Counters are indexed by position.  Tables are rebuilt when nmax changes.
-// Returns 0 on success.
+static Tuning g_tune;
+
+// Window-mode knobs of the planner (the device context's hss_tuning).
+void hc_set_tuning(uint32_t win_min_h, double win_max_frac, uint32_t tiles_per_wave) {
+    g_tune.win_min_h = win_min_h;
+    g_tune.win_max_frac = win_max_frac;
+    g_tune.tiles_per_wave = tiles_per_wave;
+}
+
+// Search root positions (roots[s] at limits[s]) with the kernels' arithmetic,
+// plain tiles as cell_kernel and window tiles as cell_win_kernel.  Counters
+// are indexed by position.  Tables are rebuilt when nmax changes.  Returns 0
+// on success.
 int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits, uint32_t n_roots, uint64_t* cand,
                     uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits,
                     hc_stats* st) {
@@ -132,18 +144,40 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
     std::string err;
     std::vector<PosReq> reqs(n_roots);
     for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limits[s], s};
-    if (!plan_batch(reqs.data(), n_roots, T.small, T.off.data(), nmax, 4096, P, err)) return 1;
-    // left sets
+    if (!plan_batch(reqs.data(), n_roots, T.small, T.off.data(), nmax, 4096, g_tune, P, err)) return 1;
+    // left sets; sorted ones (window mode) ordered by psi with the central entry last
     std::vector<i128> lre(P.left_entries), lim(P.left_entries);
     std::vector<LeftFast> lfast(P.left_entries);
+    std::vector<uint32_t> btab(P.sorted.size() * (WIN_NB + 1));
     for (size_t li = 0; li < P.lefts.size(); ++li) {
         const LeftBuild& b = P.builds[li];
         const LeftDesc& L = P.lefts[li];
+        std::vector<GInt> val(L.H);
         for (uint32_t t = 0; t < L.H; ++t) {
             uint32_t dig[MAX_SPLIT];
             decode_rank(t, b.k, b.E, dig);
             GInt z = {(i128)b.fixed, 0};
             for (uint32_t j = 0; j < b.k; ++j) z = gmul(z, gauss_value(T.pa[b.p[j]], T.pb[b.p[j]], b.E[j], dig[j]));
+            val[t] = z;
+        }
+        std::vector<uint32_t> perm(L.H);
+        for (uint32_t t = 0; t < L.H; ++t) perm[t] = t;
+        if (L.bt != NO_TABLE) {
+            const uint32_t n = L.H - 1;
+            std::vector<double> key(n);
+            for (uint32_t t = 0; t < n; ++t) key[t] = psi_of(i128_to_double(val[t].re), i128_to_double(val[t].im));
+            std::sort(perm.begin(), perm.begin() + n, [&](uint32_t x, uint32_t y) {
+                return key[x] < key[y] || (key[x] == key[y] && x < y);
+            });
+            uint32_t* bt = btab.data() + (size_t)L.bt * (WIN_NB + 1);
+            for (uint32_t bk = 0; bk <= WIN_NB; ++bk) {
+                uint32_t c = 0;
+                while (c < n && psi_bucket<WIN_NB>(key[perm[c]]) < bk) ++c;
+                bt[bk] = bk == WIN_NB ? n : c;
+            }
+        }
+        for (uint32_t t = 0; t < L.H; ++t) {
+            const GInt& z = val[perm[t]];
             lre[L.off + t] = z.re;
             lim[L.off + t] = z.im;
             LeftFast f;
@@ -162,64 +196,86 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         uint32_t left, ui, vidx, k, conj, full;
     };
     std::vector<Def> defs;
-    for (const TileDesc& Tl : P.tiles) {
-        const LeftDesc& L = P.lefts[Tl.left];
-        const PosDesc& D = P.pos[L.pos];
-        for (uint32_t k = Tl.k0; k <= Tl.k1; ++k) {
-            bool ok = !(D.l_even && !(k & 1u));
-            for (uint32_t j = 0; j < D.nsplit; ++j) ok = ok && !(k * D.split_inv[j] <= D.split_thr[j]);
-            for (uint32_t j = 0; j < D.ninert; ++j) ok = ok && !(k * D.inert_inv[j] <= D.inert_thr[j]);
-            if (!ok) continue;
-            uint32_t Hk = T.off[k + 1] - T.off[k];
-            if (L.trivial && Hk == 1) continue;
-            pairs[D.out] += 1;
-            uint64_t m = (uint64_t)L.q * k;
-            LaneCtx ctx;
-            make_lane_ctx(m, D.l4_d, D.l2_d, D.n4_d, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
-            for (uint32_t jj = 0; jj < Hk; ++jj) {
-                uint32_t vidx = T.off[k] + jj;
-                bool vc = jj + 1 == Hk;
-                double c = T.cre[vidx], d = T.cim[vidx];
-                uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
-                leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
-                for (uint32_t ui = 0; ui < L.H; ++ui) {
-                    const LeftFast& u = lfast[L.off + ui];
-                    bool uc = ui + 1 == L.H;
-                    uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                    double t1 = u.im * d, t2 = u.re * d;
-                    uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
-                    ++S.cells;
-                    for (uint32_t cj = 0; cj < 2; ++cj) {
-                        if (cj == 1 && (uc || vc)) break;
-                        double X, Y;
-                        uint32_t vX, vY;
-                        if (cj == 0) {
-                            X = __builtin_fma(u.re, c, -t1);
-                            Y = __builtin_fma(u.im, c, t2);
-                            vX = pac + QA * QA - pbd;
-                            vY = pad + pbc;
-                        } else {
-                            X = __builtin_fma(u.re, c, t1);
-                            Y = __builtin_fma(u.im, c, -t2);
-                            vX = pac + pbd;
-                            vY = pbc + QA * QA - pad;
-                        }
-
-                        LeafEval r;
-                        if (cj == 0 && uc && vc)
-                            eval_real_leaf(vX, ctx, r);  // both factors real
-                        else
-                            eval_leaf(X, Y, vX, vY, ctx, r);
-                        if (r.und) {
-                            defs.push_back({Tl.left, ui, vidx, k, cj, 1});
-                            ++S.deferred_full;
-                            continue;
-                        }
-                        cand[D.out] += r.cnt;
-                        for (int tm = 0; tm < 4; ++tm)
-                            if (r.f[tm] && root_near_integer(r.zz[tm])) {
-                                defs.push_back({Tl.left, ui, vidx, k, cj, 0});
-                                ++S.deferred_root;
+    // one leaf: u (entry ui of left set li) times v or conj(v) (cj)
+    auto leaf = [&](uint32_t li, const LeftDesc& L, const PosDesc& D, const LaneCtx& ctx, uint32_t ui, uint32_t vidx,
+                    uint32_t k, uint32_t cj, bool both_real) {
+        const LeftFast& u = lfast[L.off + ui];
+        const double c = T.cre[vidx], d = T.cim[vidx];
+        const uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
+        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+        const double t1 = u.im * d, t2 = u.re * d;
+        const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
+        ++S.cells;
+        double X, Y;
+        uint32_t vX, vY;
+        if (cj == 0) {
+            X = __builtin_fma(u.re, c, -t1);
+            Y = __builtin_fma(u.im, c, t2);
+            vX = pac + QA * QA - pbd;
+            vY = pad + pbc;
+        } else {
+            X = __builtin_fma(u.re, c, t1);
+            Y = __builtin_fma(u.im, c, -t2);
+            vX = pac + pbd;
+            vY = pbc + QA * QA - pad;
+        }
+        LeafEval r;
+        if (both_real)
+            eval_real_leaf(vX, ctx, r);
+        else
+            eval_leaf(X, Y, vX, vY, ctx, r);
+        if (r.und) {
+            defs.push_back({li, ui, vidx, k, cj, 1});
+            ++S.deferred_full;
+            return;
+        }
+        cand[D.out] += r.cnt;
+        for (int tm = 0; tm < 4; ++tm)
+            if (r.f[tm] && root_near_integer(r.zz[tm])) {
+                defs.push_back({li, ui, vidx, k, cj, 0});
+                ++S.deferred_root;
+            }
+    };
+    for (int mode = 0; mode < 2; ++mode) {
+        const std::vector<TileDesc>& tiles = mode == 0 ? P.tiles : P.wtiles;
+        for (const TileDesc& Tl : tiles) {
+            const LeftDesc& L = P.lefts[Tl.left];
+            const PosDesc& D = P.pos[L.pos];
+            S.window_tiles += mode;
+            for (uint32_t k = Tl.k0; k <= Tl.k1; ++k) {
+                bool ok = !(D.l_even && !(k & 1u));
+                for (uint32_t j = 0; j < D.nsplit; ++j) ok = ok && !(k * D.split_inv[j] <= D.split_thr[j]);
+                for (uint32_t j = 0; j < D.ninert; ++j) ok = ok && !(k * D.inert_inv[j] <= D.inert_thr[j]);
+                if (!ok) continue;
+                uint32_t Hk = T.off[k + 1] - T.off[k];
+                if (L.trivial && Hk == 1) continue;
+                pairs[D.out] += 1;
+                uint64_t m = (uint64_t)L.q * k;
+                LaneCtx ctx;
+                make_lane_ctx(m, D.l4_d, D.l2_d, D.n4_d, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
+                for (uint32_t jj = 0; jj < Hk; ++jj) {
+                    const uint32_t vidx = T.off[k] + jj;
+                    const bool vc = jj + 1 == Hk;
+                    leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
+                    // the central u: u*v only (real-leaf rule when v is real too)
+                    leaf(Tl.left, L, D, ctx, L.H - 1, vidx, k, 0, vc);
+                    if (mode == 0) {
+                        for (uint32_t ui = 0; ui + 1 < L.H; ++ui)
+                            for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) leaf(Tl.left, L, D, ctx, ui, vidx, k, cj, false);
+                        continue;
+                    }
+                    const double m2 = (double)(m * m);
+                    Window w;
+                    make_window((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, ctx.tau, 2.0 * D.l2_d * m2, w);
+                    const double pv = psi_of_f(T.cre[vidx], T.cim[vidx]);
+                    const uint32_t* bt = btab.data() + (size_t)L.bt * (WIN_NB + 1);
+                    for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
+                        uint32_t rs[4], re[4];
+                        window_ranges<WIN_NB>(w, bt, L.H - 1, pv, cj != 0, rs, re);
+                        for (int p = 0; p < 4; ++p)
+                            for (uint32_t ui = rs[p]; ui < re[p]; ++ui) {
+                                ++S.window_items;
+                                leaf(Tl.left, L, D, ctx, ui, vidx, k, cj, false);
                             }
                     }
                 }
@@ -259,6 +315,11 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
               uint64_t* leaves, uint64_t* pairs, hc_hit* hits, uint32_t hit_cap, uint32_t* n_hits, hc_stats* st) {
     std::vector<uint64_t> limits(n_roots, limit);
     return hc_search_mixed(nmax, roots, limits.data(), n_roots, cand, leaves, pairs, hits, hit_cap, n_hits, st);
+}
+
+// Mean range-window fraction of root l over partners q*k, k in [k0, k1].
+double hc_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1) {
+    return tile_window_fraction(l, limit, q, k0, k1, 33);
 }
 
 // Planner work estimate of one position (cells), or -1.

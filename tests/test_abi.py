@@ -40,8 +40,8 @@ def test_struct_layouts():
     text = HEADER.read_text()
     assert ctypes.sizeof(hss.Hit) == 40
     assert ctypes.sizeof(hss.RootResult) == 24
-    # hss_stats: 5 doubles, 6 u64, 2 u32
-    assert ctypes.sizeof(hss.Stats) == 5 * 8 + 6 * 8 + 2 * 4
+    # hss_stats: 5 doubles, 6 u64, 2 u32, 2 u64
+    assert ctypes.sizeof(hss.Stats) == 5 * 8 + 6 * 8 + 2 * 4 + 2 * 8
     for field, _ in hss.Stats._fields_:
         assert re.search(rf"\b{field}\b", text), field
 
@@ -50,7 +50,7 @@ def test_struct_layouts_match_c(tmp_path):
     """sizeof and every field offset of the ABI structs, compiled from the header
     by the host C compiler, against the ctypes binding."""
     import hss
-    structs = {"hss_hit": hss.Hit, "hss_root_result": hss.RootResult, "hss_stats": hss.Stats,
+    structs = {"hss_hit": hss.Hit, "hss_root_result": hss.RootResult, "hss_stats": hss.Stats, "hss_tuning": hss.Tuning,
                "hss_server_config": hss.ServerConfig, "hss_request_info": hss.RequestInfo,
                "hss_server_stats": hss.ServerStats}
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]

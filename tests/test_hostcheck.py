@@ -21,14 +21,37 @@ class HcHit(ctypes.Structure):
 
 class HcStats(ctypes.Structure):
     _fields_ = [("deferred_full", ctypes.c_uint64), ("deferred_root", ctypes.c_uint64),
-                ("raw_roots", ctypes.c_uint64), ("cells", ctypes.c_uint64)]
+                ("raw_roots", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("window_tiles", ctypes.c_uint64),
+                ("window_items", ctypes.c_uint64)]
+
+
+DEFAULT_TUNING = (64, 0.5, 8)  # win_min_h, win_max_frac, tiles_per_wave (hss_plan.h Tuning)
 
 
 @pytest.fixture(scope="module")
 def hc():
     if not LIB.is_file():
         subprocess.run(["make", "-C", str(PKG), "hostcheck"], check=True, capture_output=True)
-    return ctypes.CDLL(str(LIB))
+    lib = ctypes.CDLL(str(LIB))
+    lib.hc_set_tuning.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32]
+    lib.hc_set_tuning.restype = None
+    lib.hc_set_tuning(*DEFAULT_TUNING)
+    return lib
+
+
+@pytest.fixture
+def window_everywhere(hc):
+    """Window mode on every left set that can be sorted, on every tile."""
+    hc.hc_set_tuning(1, 1.0, 8)
+    yield
+    hc.hc_set_tuning(*DEFAULT_TUNING)
+
+
+@pytest.fixture
+def window_off(hc):
+    hc.hc_set_tuning(0xFFFFFFFF, 0.0, 8)
+    yield
+    hc.hc_set_tuning(*DEFAULT_TUNING)
 
 
 def hc_search(hc, nmax, limit, roots):
@@ -133,3 +156,56 @@ def test_estimate_bounds_leaves(hc):
         lv = hc_search(hc, 70000, lim, [l])[1][0]
         assert 0 < lv <= 2 * e, (l, lim, lv, e)
     assert hc.hc_estimate(ctypes.c_uint64(70000), ctypes.c_uint64(5001), ctypes.c_uint64(5000)) == -1.0
+
+
+@pytest.mark.parametrize("limit", [5000, 70000])
+def test_window_mode_per_position_golden(hc, golden, window_everywhere, limit):
+    """Every sortable left set in window mode: the per-position counts still
+    equal the reference's (only leaves that cannot count are skipped)."""
+    cases = [g for g in golden.values() if g["name"].startswith(f"l{limit}_")]
+    roots = [g["first"] for g in cases]
+    c, lv, pr, hits, st = hc_search(hc, limit, limit, roots)
+    for g, got in zip(cases, c):
+        assert got == g["candidates"], g["name"]
+    assert st.window_tiles > 0 and st.window_items > 0
+
+
+def test_window_mode_interval_1e5(hc, golden, window_everywhere):
+    for name in ("i100000_99901_100000", "i100000_50000_50099", "i100000_4380_4479"):
+        g = golden[name]
+        c = hc_search(hc, 100000, 100000, list(range(g["first"], g["last"] + 1)))[0]
+        assert sum(c) == g["candidates"], name
+
+
+def test_window_mode_full5000(hc, golden, window_everywhere):
+    g = golden["full5000"]
+    c, lv, pr, hits, st = hc_search(hc, 5000, 5000, list(range(1, 5001)))
+    assert sum(c) == g["candidates"]
+    assert sum(lv) == 170_863_629
+    assert hits == result_lines(g["results"])
+    assert st.window_tiles > 0
+
+
+def test_window_mode_skips_work_near_limit(hc, window_everywhere):
+    """Roots near LIMIT have narrow range windows: far fewer leaves evaluated
+    than enumerated, same counts as plain mode."""
+    roots = [69300, 69615, 69700, 69905]  # 69615 = 3^2 * 5 * 7 * 13 * 17
+    cw, lw, pw, hw, sw = hc_search(hc, 70000, 70000, roots)
+    hc.hc_set_tuning(0xFFFFFFFF, 0.0, 8)
+    cp, lp, pp, hp, sp = hc_search(hc, 70000, 70000, roots)
+    hc.hc_set_tuning(1, 1.0, 8)
+    assert (cw, lw, pw, hw) == (cp, lp, pp, hp)
+    assert sw.window_tiles > 0
+    assert sw.cells < 0.5 * sp.cells, (sw.cells, sp.cells)
+
+
+def test_window_fraction_bounds(hc):
+    """The planner's window measure is a fraction of the circle, 1 where the
+    range test cannot fail (small m, large LIMIT) and small near LIMIT."""
+    f = hc.hc_window_fraction
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+    assert f(10, 10_000_000, 1, 10, 20) == 1.0
+    near = f(9_990_000, 10_000_000, 1, 9_990_000, 10_000_000)
+    assert 0 < near < 0.02
+    assert 0 < f(7_500_000, 10_000_000, 1, 7_500_000, 10_000_000) < 0.6
