@@ -12,10 +12,14 @@ resident in HBM (built once, outside the timed region).  Every step's total
 candidate count is checked against the first step's, and the slice's total
 against the reference's 39,091,522,776 (bit-exact or the run fails).
 
-Prints one JSON line.  roofline: the cell kernel is bound by the vector ALU
-(integer and FP64 lane operations), not by HBM or MFMA; achieved uses the
-algorithmic work 64 * leaves + 32 * candidates lane-ops per launch divided by
-the kernel's HIP-event duration; peak = 256 CU x 64 lanes x 2.4 GHz.
+Prints one JSON line.  roofline: the enumeration runs as two persistent
+kernels, cell_win_kernel (tiles whose range window is narrow: only leaves that
+can pass the range test are evaluated) and cell_kernel (every leaf); the line
+describes the one with the longer HIP-event duration.  Both are bound by the
+vector ALU (integer and FP64 lane operations), not by HBM or MFMA; achieved =
+64 lane-ops per evaluated leaf (the fast path's essential arithmetic) times
+the kernel's evaluated leaves per launch, divided by its duration; peak = 256
+CU x 64 lanes x 2.4 GHz.
 cpu_baseline: the reference search_v18 compiled from source (oracle/_ref/search),
 timed on this host on a bounded sample of the same workload and scaled by the
 exact leaf counts to the whole batch.
@@ -44,23 +48,24 @@ METRIC = "verified root-side positions/sec at LIMIT=1e7 (bit-exact); p50/p99 req
 LIMIT = 10_000_000
 SLICE_CANDIDATES = 39_091_522_776  # evidence/benchmarks/gpu-opt-benchmark.json:15-20
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # lane-ops per second, in T
+OPS_PER_LEAF = 64  # lane-ops of one evaluated leaf's fast-path arithmetic
 
 
-def profiled_traffic(leaves: int) -> int | None:
-    """HBM bytes per launch measured by the committed rocprofv3 counter passes
-    (profiles/cell_kernel_uneven1536_counters.md), when they describe this
-    exact launch (same leaf count); None otherwise."""
-    path = ROOT / "profiles" / "cell_kernel_uneven1536_counters.md"
+def profiled_traffic(kernel: str, evaluated: int) -> int | None:
+    """HBM bytes per launch of `kernel` measured by the committed rocprofv3
+    counter passes (profiles/cell_kernels_uneven1536_counters.md), when they
+    describe this exact launch (same kernel, same evaluated-leaf count)."""
+    path = ROOT / "profiles" / "cell_kernels_uneven1536_counters.md"
     if not path.is_file():
         return None
     kv = {}
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_") and ":" in line:
+        if line.startswith(f"traffic.{kernel}.") and ":" in line:
             k, v = line.split(":", 1)
-            kv[k.strip()] = int(v.strip())
-    if kv.get("traffic_leaves_per_launch") != leaves:
+            kv[k.strip().split(".")[-1]] = int(v.strip())
+    if kv.get("evaluated_per_launch") != evaluated:
         return None
-    return kv.get("traffic_bytes_per_launch")
+    return kv.get("bytes_per_launch")
 
 
 def plain_form(ctx, roots: list[int]) -> dict:
@@ -201,7 +206,7 @@ def main() -> int:
     if slice_total is not None and slice_total != SLICE_CANDIDATES:
         log(f"PARITY FAILURE: slice candidates {slice_total} != {SLICE_CANDIDATES}")
         return 1
-    cells_ms, exact_ms, stats = [], [], None
+    cells_ms, exact_ms, win_ms, plain_ms, stats = [], [], [], [], None
     t_start = time.perf_counter()
     for i in range(args.steps):
         r = step()
@@ -212,6 +217,8 @@ def main() -> int:
             return 1
         cells_ms.append(r.stats["cells_ms"])
         exact_ms.append(r.stats["exact_ms"])
+        win_ms.append(r.stats["window_ms"])
+        plain_ms.append(r.stats["plain_ms"])
         stats = r.stats
         log(f"step {i}: {r.stats['total_ms']:.1f} ms (cells {r.stats['cells_ms']:.1f} ms)")
     elapsed = time.perf_counter() - t_start
@@ -221,8 +228,12 @@ def main() -> int:
     value = positions / elapsed
     leaves, cands = stats["leaves"], stats["candidates"]
     cell_s = sum(cells_ms) / len(cells_ms) / 1e3
-    ops = 64 * leaves + 32 * cands
-    achieved = ops / cell_s / 1e12
+    w_ms, p_ms = sum(win_ms) / len(win_ms), sum(plain_ms) / len(plain_ms)
+    w_eval = stats["window_evaluated"]
+    p_eval = stats["evaluated"] - w_eval
+    kernel, k_ms, k_eval = ("cell_win_kernel", w_ms, w_eval) if w_ms >= p_ms else ("cell_kernel", p_ms, p_eval)
+    ops = OPS_PER_LEAF * k_eval
+    achieved = ops / (k_ms / 1e3) / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -240,9 +251,13 @@ def main() -> int:
                                "(slice 7500000-7500511 + 512 heavy + 512 light roots, seed 20261203)",
                    "limit": LIMIT, "positions": len(roots), "parallelism": "1 GPU, persistent grid"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": profiled_traffic(leaves),
-                     "kernel": "cell_kernel", "ops_per_launch": ops, "kernel_ms": cell_s * 1e3},
-        "work": {"leaves": leaves, "candidates": cands, "tiles": stats["tiles"], "deferred": stats["deferred"],
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": profiled_traffic(kernel, k_eval),
+                     "kernel": kernel, "kernel_ms": k_ms, "evaluated_leaves": k_eval, "ops_per_leaf": OPS_PER_LEAF,
+                     "ops_per_launch": ops},
+        "work": {"leaves": leaves, "evaluated": stats["evaluated"], "candidates": cands, "tiles": stats["tiles"],
+                 "window_tiles": stats["window_tiles"], "window_ms": w_ms, "window_evaluated": w_eval,
+                 "plain_ms": p_ms, "plain_evaluated": p_eval, "deferred": stats["deferred"],
+                 "setup_ms": stats["setup_ms"], "plan_ms": stats["plan_ms"],
                  "exact_ms": sum(exact_ms) / len(exact_ms), "leaves_per_s": leaves / cell_s},
     }
     if args.plain:

@@ -62,9 +62,9 @@ struct Counters {
 struct LaunchBuffers {
     DevBuf pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
     HostBuf upload;   // pinned staging of the plan
-    HostBuf h_misc;   // pinned: [0] tile head, [1] deferred, [2] raw, [3] hits
+    HostBuf h_misc;   // pinned copy of misc: [1] deferred, [2] raw, [3] hits, [6..9] evaluated (2 x u64)
     uint32_t def_cap = 0;
-    hipEvent_t ev[4] = {};  // start, setup done, cells done, end
+    hipEvent_t ev[5] = {};  // start, setup done, window cells done, plain cells done, end
     Plan plan;
     uint32_t npos = 0;
     static constexpr uint32_t RAW_CAP = 1u << 16, HIT_CAP = 1u << 16;
@@ -78,7 +78,10 @@ struct LaunchResult {
     uint32_t deferred = 0;
     uint32_t raw = 0;
     std::vector<HitOut> hits;  // accepted (not de-duplicated), HitOut::pos = plan position
+    uint64_t evaluated = 0;    // leaves evaluated by the fast path
+    uint64_t win_evaluated = 0, plain_evaluated = 0;
     float setup_ms = 0, cells_ms = 0, exact_ms = 0;
+    float win_ms = 0, plain_ms = 0;  // the two parts of cells_ms
 };
 
 }  // namespace hss
@@ -125,7 +128,7 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n);
 // the positions' counters, cells, exact and acceptance kernels, read-back of
 // the misc counters into pinned memory.  Asynchronous.
 int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
-// After B.ev[3] has completed: read the misc counters, detect overflow, copy
+// After B.ev[4] has completed: read the misc counters, detect overflow, copy
 // accepted hits on `copy` (a stream with no work that writes B's buffers).
 int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out);
 

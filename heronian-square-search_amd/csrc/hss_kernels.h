@@ -37,6 +37,7 @@ struct CellParams {
     unsigned long long* cand;    // per counter index (PosDesc::out)
     unsigned long long* leaves;  // per counter index
     unsigned long long* pairs;   // per counter index: pairs reaching the enumeration
+    unsigned long long* evaluated;  // leaves evaluated by the fast path: [0] window kernel, [1] plain kernel
     DefItem* def;     // leaves for the exact kernel
     uint32_t* ndef;
     uint32_t def_cap;

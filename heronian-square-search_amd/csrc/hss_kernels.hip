@@ -525,6 +525,7 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
         atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
         atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
         atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
+        atomicAdd(P.evaluated + 1, lane == 0 ? (unsigned long long)leaves : 0ull);  // every leaf is evaluated
     }
 }
 
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
         const PosDesc& S = P.pos[L.pos];
         const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
         const uint32_t nsort = L.H - 1;
-        uint64_t cand = 0, leaves = 0, pairs = 0;
+        uint64_t cand = 0, leaves = 0, pairs = 0, evald = 0;
         for (uint32_t par = 0; par < 2; ++par) {
             uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
             for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
@@ -632,6 +633,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
                     n0 = act ? n0 : 0u;
                     n1 = (act && !vc) ? n1 : 0u;
                     const uint32_t cnt = n0 + n1;
+                    evald += act ? 1u + cnt : 0u;  // the central leaf and the items
                     __builtin_amdgcn_wave_barrier();
                     WinRec& R = rec[lane];
                     R.c = c;
@@ -718,9 +720,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
         cand = wave_sum64(cand);
         leaves = wave_sum64(leaves);
         pairs = wave_sum64(pairs);
+        evald = wave_sum64(evald);
         atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
         atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
         atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
+        atomicAdd(P.evaluated, lane == 0 ? (unsigned long long)evald : 0ull);
     }
 }
 
@@ -766,12 +770,12 @@ __global__ __launch_bounds__(256) void k_accept(const PosDesc* pos, const HitOut
 }
 
 // Start of a batch: zero the work-queue heads and the deferred / raw / hit
-// counters (misc[0..7]) and the counters of the batch's positions.  A kernel
+// counters (misc[0..15]) and the counters of the batch's positions.  A kernel
 // rather than memset nodes, so the launch sequence can be captured in a graph.
 __global__ void k_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand,
                               unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 8) misc[i] = 0;
+    if (i < 16) misc[i] = 0;
     for (; i < npos; i += gridDim.x * blockDim.x) {
         const uint32_t o = pos[i].out;
         cand[o] = 0;
@@ -870,7 +874,7 @@ hipError_t launch_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* 
 hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand, unsigned long long* leaves,
                               unsigned long long* pairs, uint32_t* misc, hipStream_t st) {
     uint32_t blocks = (npos + 255) / 256;
-    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);  // >= 1 block: misc is always cleared
     hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, st, pos, npos, cand, leaves, pairs, misc);
     return hipGetLastError();
 }

@@ -386,6 +386,7 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.cand = cnt.cand;
     K.leaves = cnt.leaves;
     K.pairs = cnt.pairs;
+    K.evaluated = reinterpret_cast<unsigned long long*>(misc + 6);
     K.def = B.def.as<DefItem>();
     K.ndef = misc + 1;
     K.def_cap = B.def_cap;
@@ -396,15 +397,17 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     HIP_TRY(hipEventRecord(B.ev[1], st));
     vlog(st, "left sets built");
     if (nwtiles) HIP_TRY(launch_cells_win(K, c->win_grid_blocks(), st));
+    vlog(st, "window cell kernel");
+    HIP_TRY(hipEventRecord(B.ev[2], st));
     if (ntiles) HIP_TRY(launch_cells(K, c->grid_blocks(), st));
     vlog(st, "cell kernel");
-    HIP_TRY(hipEventRecord(B.ev[2], st));
+    HIP_TRY(hipEventRecord(B.ev[3], st));
     HIP_TRY(launch_exact(K, (uint32_t)c->cus * 4, st));
     HIP_TRY(launch_accept(K.pos, K.raw, K.nraw, LaunchBuffers::RAW_CAP, B.hits.as<HitOut>(), misc + 3,
                           LaunchBuffers::HIT_CAP, st));
     vlog(st, "exact + accept");
-    HIP_TRY(hipMemcpyAsync(B.h_misc.p, misc, 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipEventRecord(B.ev[3], st));
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, misc, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(B.ev[4], st));
     return HSS_OK;
 }
 
@@ -413,9 +416,14 @@ int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult&
     const uint32_t* m4 = B.h_misc.as<uint32_t>();
     out.deferred = m4[1];
     out.raw = m4[2];
+    std::memcpy(&out.win_evaluated, m4 + 6, 8);
+    std::memcpy(&out.plain_evaluated, m4 + 8, 8);
+    out.evaluated = out.win_evaluated + out.plain_evaluated;
     (void)hipEventElapsedTime(&out.setup_ms, B.ev[0], B.ev[1]);
-    (void)hipEventElapsedTime(&out.cells_ms, B.ev[1], B.ev[2]);
-    (void)hipEventElapsedTime(&out.exact_ms, B.ev[2], B.ev[3]);
+    (void)hipEventElapsedTime(&out.win_ms, B.ev[1], B.ev[2]);
+    (void)hipEventElapsedTime(&out.plain_ms, B.ev[2], B.ev[3]);
+    (void)hipEventElapsedTime(&out.exact_ms, B.ev[3], B.ev[4]);
+    out.cells_ms = out.win_ms + out.plain_ms;
     if (m4[1] > B.def_cap) {
         // deferred buffer overflowed: the counts are incomplete; grow for a rerun
         if (m4[1] > (1u << 30)) return fail(HSS_ENOMEM, "deferred-leaf buffer overflow");
@@ -465,7 +473,7 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     for (int attempt = 0;; ++attempt) {
         rc = launch_enqueue(c, B, cnt, c->stream);
         if (rc != HSS_OK) return rc;
-        HIP_TRY(hipEventSynchronize(B.ev[3]));
+        HIP_TRY(hipEventSynchronize(B.ev[4]));
         rc = launch_collect(c, B, c->stream, R);
         if (rc != HSS_OK) return rc;
         if (!R.rerun) break;
@@ -485,6 +493,10 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     c->stats.exact_ms = R.exact_ms;
     c->stats.tiles = B.plan.tiles.size() + B.plan.wtiles.size();
     c->stats.window_tiles = B.plan.wtiles.size();
+    c->stats.evaluated = R.evaluated;
+    c->stats.window_evaluated = R.win_evaluated;
+    c->stats.window_ms = R.win_ms;
+    c->stats.plain_ms = R.plain_ms;
     c->stats.sorted_lefts = B.plan.sorted.size();
     c->stats.left_entries = B.plan.left_entries;
     c->stats.deferred = R.deferred;

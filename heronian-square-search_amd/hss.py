@@ -40,7 +40,9 @@ class Stats(ctypes.Structure):
                 ("left_entries", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("raw_roots", ctypes.c_uint64),
                 ("leaves", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
                 ("block_threads", ctypes.c_uint32), ("window_tiles", ctypes.c_uint64),
-                ("sorted_lefts", ctypes.c_uint64)]
+                ("sorted_lefts", ctypes.c_uint64), ("evaluated", ctypes.c_uint64), ("window_evaluated", ctypes.c_uint64),
+                ("window_ms", ctypes.c_double),
+                ("plain_ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

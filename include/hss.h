@@ -65,6 +65,10 @@ typedef struct {
     uint32_t block_threads;
     uint64_t window_tiles; /* tiles run by the window-mode kernel (range windows) */
     uint64_t sorted_lefts; /* left sets stored sorted by angle for it */
+    uint64_t evaluated;    /* leaves evaluated by the fast path (leaves minus those skipped) */
+    uint64_t window_evaluated; /* ... of them by the window-mode kernel */
+    double window_ms;      /* the window-mode kernel's part of cells_ms */
+    double plain_ms;       /* the plain kernel's part of cells_ms */
 } hss_stats;
 
 /*
