@@ -174,6 +174,8 @@ def main() -> int:
     ap.add_argument("--plain", action="store_true",
                     help="also time the one-interval-per-launch form of the same roots")
     ap.add_argument("--no-served", action="store_true", help="skip the served request stream")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner on calibration roots (seed+1 draws, not the measured ones) first")
     ap.add_argument("--served-requests", type=int, default=40)
     ap.add_argument("--served-rate", type=float, default=5.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
@@ -191,6 +193,13 @@ def main() -> int:
     ctx = hss.Context(LIMIT, hss.visible_device())
     log(f"tables for N_max={LIMIT} built in {time.perf_counter() - t0:.2f} s "
         f"({ctx.device_bytes / 2**30:.2f} GiB resident)")
+
+    if args.autotune:
+        heavy, light = workloads.uneven_roots(seed=workloads.SEED + 1)
+        calib = heavy[::8] + light[::16] + list(range(7_400_000, 7_400_064))
+        t0 = time.perf_counter()
+        tuned, ms = ctx.autotune(LIMIT, calib, repeats=2)
+        log(f"autotune on {len(calib)} calibration roots: {tuned} ({ms:.1f} ms, {time.perf_counter() - t0:.1f} s)")
 
     def step():
         r = ctx.search(LIMIT, roots)
@@ -260,6 +269,7 @@ def main() -> int:
                  "setup_ms": stats["setup_ms"], "plan_ms": stats["plan_ms"],
                  "exact_ms": sum(exact_ms) / len(exact_ms), "leaves_per_s": leaves / cell_s},
     }
+    line["tuning"] = ctx.get_tuning()
     if args.plain:
         log("timing the plain one-interval-per-launch form")
         pf = plain_form(ctx, sorted(roots))

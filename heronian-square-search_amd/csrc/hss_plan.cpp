@@ -211,13 +211,16 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
     const uint32_t nl = (uint32_t)P.lefts.size();
     std::vector<uint8_t> eligible(nl, 0);
     std::vector<double> gain(nl, 1.0);  // cost per cell of the whole left set
+    std::vector<double> fmax(nl, 0.0);
     for (uint32_t li = 0; li < nl; ++li) {
         const LeftDesc& L = P.lefts[li];
         const uint32_t H = L.H;
-        if (H < tune.win_min_h || H < WIN_MIN_SORT + 1 || H > WIN_MAX_SORT + 1) continue;
+        const int wc = weight_class(H);
+        if (wc < 0 || !(tune.win_max_frac[wc] > 0.0)) continue;
         eligible[li] = 1;
+        fmax[li] = tune.win_max_frac[wc];
         const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, P.kmin[li], P.kmax[li], 9);
-        if (f <= tune.win_max_frac) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
+        if (f <= fmax[li]) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
         if (gain[li] > 1.0) gain[li] = 1.0;
     }
     double total = 0;
@@ -255,7 +258,7 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
             bool win = false;
             if (eligible[li]) {
                 const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);
-                if (f <= tune.win_max_frac) {
+                if (f <= fmax[li]) {
                     win = true;
                     cost = cells * f * tune.win_item_cost +
                            (double)entries(off, k0, kend) * tune.win_partner_cost;

@@ -21,17 +21,29 @@
 
 namespace hss {
 
-// Scheduling knobs of one batch.  A tile runs in window mode (cell_win_kernel)
-// when its left set has win_min_h <= H <= WIN_MAX_SORT + 1 entries and its
-// mean range-window fraction is at most win_max_frac; otherwise in plain mode
-// (cell_kernel).  Keyed on H, i.e. on the Gaussian weight of l and of the
-// shared prime powers: H = (w(l; E) + 1) / 2.
+// Scheduling knobs of one batch, keyed on Gaussian weight.  A left set of
+// H = (w + 1) / 2 entries (w the Gaussian weight of l with the shared prime
+// powers merged) falls in weight class floor(log2 H) - 3 (H in [9, 4097]:
+// classes 0..8; smaller or larger left sets never use window mode).  A tile
+// runs in window mode (cell_win_kernel) when the mean range-window fraction of
+// its partner range is at most win_max_frac[class]; otherwise in plain mode
+// (cell_kernel).  Counts never depend on these values, only speed does.
+static constexpr int WEIGHT_CLASSES = 9;
+
+inline int weight_class(uint32_t H) {
+    if (H < WIN_MIN_SORT + 1 || H > WIN_MAX_SORT + 1) return -1;
+    int c = 31 - __builtin_clz(H) - 3;
+    return c < 0 ? 0 : (c >= WEIGHT_CLASSES ? WEIGHT_CLASSES - 1 : c);
+}
+
+// Defaults: hss_autotune on MI355X, calibration roots drawn like the uneven
+// LIMIT 1e7 workload with another seed (tiles_per_wave 32 measured 1% faster
+// than 16 but plans 35% longer on the host).
 struct Tuning {
-    uint32_t win_min_h = 64;
-    double win_max_frac = 0.5;
-    double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf
-    double win_partner_cost = 4;  // per partner entry, in plain-leaf units
-    uint32_t tiles_per_wave = 8;  // tile size: est. work / (waves * tiles_per_wave)
+    uint32_t tiles_per_wave = 16;  // tile size: est. work / (waves * tiles_per_wave)
+    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.3, 0.2, 0.5, 0.6, 0.5, 0.5, 0.5};
+    double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf (tile order)
+    double win_partner_cost = 4;  // per partner entry of a window tile, in plain-leaf units
 };
 
 struct Plan {

@@ -268,9 +268,9 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
 
 extern "C" int hss_ctx_get_tuning(const hss_ctx* c, hss_tuning* out) {
     if (!c || !out) return fail(HSS_EINVAL, "null argument");
-    out->win_min_h = c->tune.win_min_h;
+    *out = hss_tuning{};
     out->tiles_per_wave = c->tune.tiles_per_wave;
-    out->win_max_frac = c->tune.win_max_frac;
+    for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i) out->win_max_frac[i] = c->tune.win_max_frac[i];
     out->win_item_cost = c->tune.win_item_cost;
     out->win_partner_cost = c->tune.win_partner_cost;
     return HSS_OK;
@@ -280,13 +280,105 @@ extern "C" int hss_ctx_set_tuning(hss_ctx* c, const hss_tuning* in) {
     if (!c || !in) return fail(HSS_EINVAL, "null argument");
     if (c->server) return fail(HSS_EINVAL, "context is attached to a server");
     if (!in->tiles_per_wave || in->tiles_per_wave > 1024) return fail(HSS_EINVAL, "tiles_per_wave outside [1, 1024]");
-    if (!(in->win_max_frac >= 0.0 && in->win_max_frac <= 1.0)) return fail(HSS_EINVAL, "win_max_frac outside [0, 1]");
+    for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i)
+        if (!(in->win_max_frac[i] >= 0.0 && in->win_max_frac[i] <= 1.0))
+            return fail(HSS_EINVAL, "win_max_frac outside [0, 1]");
     if (!(in->win_item_cost > 0.0) || !(in->win_partner_cost >= 0.0)) return fail(HSS_EINVAL, "bad cost weights");
-    c->tune.win_min_h = in->win_min_h;
     c->tune.tiles_per_wave = in->tiles_per_wave;
-    c->tune.win_max_frac = in->win_max_frac;
+    for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i) c->tune.win_max_frac[i] = in->win_max_frac[i];
     c->tune.win_item_cost = in->win_item_cost;
     c->tune.win_partner_cost = in->win_partner_cost;
+    return HSS_OK;
+}
+
+extern "C" int hss_autotune(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint32_t repeats,
+                            hss_tuning* out, double* best_ms) {
+    if (!c || (!roots && n_roots)) return fail(HSS_EINVAL, "null argument");
+    if (!n_roots) return fail(HSS_EINVAL, "no calibration roots");
+    if (c->server) return fail(HSS_EINVAL, "context is attached to a server");
+    if (!repeats) repeats = 1;
+    const Tuning start = c->tune;
+    uint64_t ref_total = 0;
+    bool have_ref = false;
+    // device time (setup + cells + exact) of the calibration batch under t, best of `repeats`
+    auto measure = [&](const Tuning& t, double& ms) -> int {
+        c->tune = t;
+        ms = 1e300;
+        for (uint32_t r = 0; r < repeats; ++r) {
+            uint64_t total = 0;
+            int rc = hss_search(c, limit, roots, n_roots, nullptr, &total, nullptr, 0, nullptr);
+            if (rc != HSS_OK && rc != HSS_EOVERFLOW) return rc;
+            if (!have_ref) {
+                ref_total = total;
+                have_ref = true;
+            } else if (total != ref_total) {
+                return fail(HSS_EDEVICE, "autotune: a tuning changed the candidate count");
+            }
+            ms = std::min(ms, c->stats.setup_ms + c->stats.cells_ms + c->stats.exact_ms);
+        }
+        return HSS_OK;
+    };
+    // weight classes in order of their share of the batch's cells
+    double share[WEIGHT_CLASSES] = {0};
+    {
+        std::vector<PosReq> reqs(n_roots);
+        for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limit, s};
+        Plan P;
+        std::string err;
+        if (!plan_batch(reqs.data(), n_roots, c->small_primes, c->h_off.data(), c->nmax, c->waves(), start, P, err))
+            return fail(HSS_EINVAL, err);
+        for (size_t li = 0; li < P.lefts.size(); ++li) {
+            const int wc = weight_class(P.lefts[li].H);
+            if (wc >= 0)
+                share[wc] += (double)P.lefts[li].H * ((double)c->h_off[P.kmax[li] + 1] - c->h_off[P.kmin[li]]);
+        }
+    }
+    std::vector<int> order;
+    for (int i = 0; i < WEIGHT_CLASSES; ++i)
+        if (share[i] > 0) order.push_back(i);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return share[a] > share[b]; });
+    Tuning best = start;
+    double best_t = 0;
+    int rc = measure(best, best_t);
+    if (rc != HSS_OK) {
+        c->tune = start;
+        return rc;
+    }
+    static const double grid[] = {0.0, 0.05, 0.1, 0.2, 0.3, 0.45, 0.6, 0.8, 1.0};
+    for (int wc : order)
+        for (double g : grid) {
+            if (g == best.win_max_frac[wc]) continue;
+            Tuning t = best;
+            t.win_max_frac[wc] = g;
+            double ms;
+            rc = measure(t, ms);
+            if (rc != HSS_OK) {
+                c->tune = start;
+                return rc;
+            }
+            if (ms < best_t) {
+                best_t = ms;
+                best = t;
+            }
+        }
+    for (uint32_t tpw : {4u, 8u, 16u, 32u}) {
+        if (tpw == best.tiles_per_wave) continue;
+        Tuning t = best;
+        t.tiles_per_wave = tpw;
+        double ms;
+        rc = measure(t, ms);
+        if (rc != HSS_OK) {
+            c->tune = start;
+            return rc;
+        }
+        if (ms < best_t) {
+            best_t = ms;
+            best = t;
+        }
+    }
+    c->tune = best;
+    if (out) (void)hss_ctx_get_tuning(c, out);
+    if (best_ms) *best_ms = best_t;
     return HSS_OK;
 }
 

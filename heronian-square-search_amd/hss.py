@@ -48,15 +48,27 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+WEIGHT_CLASSES = 9
+
+
 class Tuning(ctypes.Structure):
-    _fields_ = [("win_min_h", ctypes.c_uint32), ("tiles_per_wave", ctypes.c_uint32), ("win_max_frac", ctypes.c_double),
-                ("win_item_cost", ctypes.c_double), ("win_partner_cost", ctypes.c_double)]
+    _fields_ = [("tiles_per_wave", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("win_max_frac", ctypes.c_double * WEIGHT_CLASSES), ("win_item_cost", ctypes.c_double),
+                ("win_partner_cost", ctypes.c_double)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {"tiles_per_wave": self.tiles_per_wave, "win_max_frac": list(self.win_max_frac),
+                "win_item_cost": self.win_item_cost, "win_partner_cost": self.win_partner_cost}
 
-
-WINDOW_OFF = 0xFFFFFFFF
+    @classmethod
+    def from_dict(cls, d: dict) -> "Tuning":
+        t = cls()
+        t.tiles_per_wave = d["tiles_per_wave"]
+        for i, v in enumerate(d["win_max_frac"]):
+            t.win_max_frac[i] = v
+        t.win_item_cost = d["win_item_cost"]
+        t.win_partner_cost = d["win_partner_cost"]
+        return t
 
 
 class ServerConfig(ctypes.Structure):
@@ -109,6 +121,8 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.hss_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.hss_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     lib.hss_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.hss_autotune.argtypes = [vp, u64, ctypes.POINTER(u64), u32, u32, ctypes.POINTER(Tuning),
+                                 ctypes.POINTER(ctypes.c_double)]
     lib.hss_verify_triple.argtypes = [u64, u64, u64, u64, u64]
     lib.hss_estimate.argtypes = [vp, u64, u64, u64, ctypes.POINTER(ctypes.c_double)]
     lib.hss_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -198,12 +212,26 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, **kw) -> dict:
-        """Change scheduling knobs (counts never depend on them); returns the old ones."""
+        """Change scheduling knobs (counts never depend on them); returns the old
+        ones.  win_max_frac may be a list (one per weight class) or one number
+        for every class."""
         old = self.get_tuning()
         new = dict(old, **kw)
-        t = Tuning(**new)
+        if not isinstance(new["win_max_frac"], (list, tuple)):
+            new["win_max_frac"] = [float(new["win_max_frac"])] * WEIGHT_CLASSES
+        t = Tuning.from_dict(new)
         _check(_lib.hss_ctx_set_tuning(self._h, ctypes.byref(t)))
         return old
+
+    def autotune(self, limit: int, roots, repeats: int = 1) -> tuple[dict, float]:
+        """Run the autotuner on calibration roots; applies and returns the best
+        tuning and its device time (ms)."""
+        roots = [int(r) for r in roots]
+        arr = (ctypes.c_uint64 * len(roots))(*roots)
+        t = Tuning()
+        ms = ctypes.c_double()
+        _check(_lib.hss_autotune(self._h, limit, arr, len(roots), repeats, ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
 
     def search(self, limit: int, roots, hit_cap: int = 4096) -> SearchResult:
         roots = [int(r) for r in roots]

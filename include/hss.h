@@ -114,24 +114,40 @@ int hss_search_interval(hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t l
 
 /*
  * Scheduling knobs (no counterpart in the reference, whose kernel has one
- * fixed launch shape, src/gpu_search_v42.cu:457-458).  A tile -- one left set
- * of a root against a range of partners -- runs in window mode (only the left
- * entries whose leaves can pass the range test, found from angle-sorted left
- * sets) when the left set has at least win_min_h half-set entries (H =
- * (w + 1) / 2 for its Gaussian weight w) and its mean range-window fraction is
- * at most win_max_frac; otherwise every leaf is evaluated.  Counts never
- * depend on these values; only speed does.
+ * fixed launch shape, src/gpu_search_v42.cu:457-458), keyed on Gaussian
+ * weight.  A tile -- one left set of a root against a range of partners --
+ * runs in window mode (only the left entries whose leaves can pass the range
+ * test, found from angle-sorted left sets) when the mean range-window fraction
+ * of its partners is at most win_max_frac[c], c the weight class of the left
+ * set: H = (w + 1) / 2 entries for its Gaussian weight w, c = floor(log2 H) - 3
+ * for 9 <= H <= 4097 (2^(c+3) <= H < 2^(c+4); class 8 up to 4097); other left
+ * sets, and classes with 0, evaluate every leaf.  Counts never depend on these
+ * values; only speed does.
  */
+#define HSS_WEIGHT_CLASSES 9
+
 typedef struct {
-    uint32_t win_min_h;      /* 0xFFFFFFFF: window mode off */
     uint32_t tiles_per_wave; /* tile size: estimated work / (resident waves * this) */
-    double win_max_frac;
+    uint32_t reserved;
+    double win_max_frac[HSS_WEIGHT_CLASSES];
     double win_item_cost;    /* planning cost of a window-mode leaf vs a plain leaf */
     double win_partner_cost; /* planning cost per partner entry of a window-mode tile */
 } hss_tuning;
 
 int hss_ctx_get_tuning(const hss_ctx* ctx, hss_tuning* out);
 int hss_ctx_set_tuning(hss_ctx* ctx, const hss_tuning* in);
+
+/*
+ * Autotuner: searches win_max_frac per weight class (coordinate descent over
+ * {0, 0.05, 0.1, 0.2, 0.3, 0.45, 0.6, 0.8, 1}, classes in order of their share
+ * of the work) and then tiles_per_wave over {4, 8, 16, 32}, timing the device
+ * part of hss_search (HIP events, best of `repeats`) on the calibration roots
+ * at `limit`.  Every setting must give the same candidate total (else
+ * HSS_EDEVICE).  The best tuning is applied to ctx and copied to *out
+ * (nullable); *best_ms (nullable) receives its time.
+ */
+int hss_autotune(hss_ctx* ctx, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint32_t repeats,
+                 hss_tuning* out, double* best_ms);
 
 /* Timings and work counters of the last hss_search / hss_search_interval call. */
 int hss_get_stats(const hss_ctx* ctx, hss_stats* out);

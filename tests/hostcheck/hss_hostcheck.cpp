@@ -124,10 +124,10 @@ typedef struct {
 
 static Tuning g_tune;
 
-// Window-mode knobs of the planner (the device context's hss_tuning).
-void hc_set_tuning(uint32_t win_min_h, double win_max_frac, uint32_t tiles_per_wave) {
-    g_tune.win_min_h = win_min_h;
-    g_tune.win_max_frac = win_max_frac;
+// Window-mode knobs of the planner (the device context's hss_tuning): weight
+// classes >= first_class get threshold frac, the others 0 (window mode off).
+void hc_set_tuning(double frac, uint32_t first_class, uint32_t tiles_per_wave) {
+    for (int i = 0; i < WEIGHT_CLASSES; ++i) g_tune.win_max_frac[i] = (uint32_t)i >= first_class ? frac : 0.0;
     g_tune.tiles_per_wave = tiles_per_wave;
 }
 

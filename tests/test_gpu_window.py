@@ -16,8 +16,8 @@ from conftest import result_lines
 pytestmark = pytest.mark.gpu
 
 MODES = {
-    "everywhere": dict(win_min_h=1, win_max_frac=1.0),
-    "off": dict(win_min_h=0xFFFFFFFF),
+    "everywhere": dict(win_max_frac=1.0),
+    "off": dict(win_max_frac=0.0),
 }
 
 
@@ -129,4 +129,23 @@ def test_tuning_validation(hss_mod, ctx_small):
         with pytest.raises(hss_mod.HssError):
             ctx_small.set_tuning(**bad)
     t = ctx_small.get_tuning()
-    assert t["tiles_per_wave"] >= 1 and 0.0 <= t["win_max_frac"] <= 1.0
+    assert t["tiles_per_wave"] >= 1 and all(0.0 <= f <= 1.0 for f in t["win_max_frac"])
+
+
+def test_autotune_keeps_counts(hss_mod, ctx_big):
+    """The autotuner times window thresholds per weight class and tile sizes on
+    calibration roots; every setting it tries must give the same candidate
+    total (it fails otherwise), and the tuning it applies still gives the
+    reference's counts."""
+    import workloads
+    heavy, light = workloads.uneven_roots(seed=workloads.SEED + 1)
+    calib = heavy[::32] + light[::64] + list(range(7_400_000, 7_400_016))
+    old = ctx_big.get_tuning()
+    try:
+        tuned_t, ms = ctx_big.autotune(10_000_000, calib, repeats=1)
+        print("autotuned", tuned_t, ms, flush=True)
+        assert ms > 0 and tuned_t == ctx_big.get_tuning()
+        tot, _ = ctx_big.search_interval(10_000_000, 7_500_000, 7_500_127)
+        assert tot == 10_369_431_230  # evidence/benchmarks/v45-ab.json:11
+    finally:
+        ctx_big.set_tuning(**old)

@@ -25,7 +25,7 @@ class HcStats(ctypes.Structure):
                 ("window_items", ctypes.c_uint64)]
 
 
-DEFAULT_TUNING = (64, 0.5, 8)  # win_min_h, win_max_frac, tiles_per_wave (hss_plan.h Tuning)
+DEFAULT_TUNING = (0.5, 3, 8)  # threshold for weight classes >= 3 (H >= 64), tiles_per_wave (hss_plan.h Tuning)
 
 
 @pytest.fixture(scope="module")
@@ -33,7 +33,7 @@ def hc():
     if not LIB.is_file():
         subprocess.run(["make", "-C", str(PKG), "hostcheck"], check=True, capture_output=True)
     lib = ctypes.CDLL(str(LIB))
-    lib.hc_set_tuning.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32]
+    lib.hc_set_tuning.argtypes = [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32]
     lib.hc_set_tuning.restype = None
     lib.hc_set_tuning(*DEFAULT_TUNING)
     return lib
@@ -42,14 +42,14 @@ def hc():
 @pytest.fixture
 def window_everywhere(hc):
     """Window mode on every left set that can be sorted, on every tile."""
-    hc.hc_set_tuning(1, 1.0, 8)
+    hc.hc_set_tuning(1.0, 0, 8)
     yield
     hc.hc_set_tuning(*DEFAULT_TUNING)
 
 
 @pytest.fixture
 def window_off(hc):
-    hc.hc_set_tuning(0xFFFFFFFF, 0.0, 8)
+    hc.hc_set_tuning(0.0, 0, 8)
     yield
     hc.hc_set_tuning(*DEFAULT_TUNING)
 
@@ -191,9 +191,9 @@ def test_window_mode_skips_work_near_limit(hc, window_everywhere):
     than enumerated, same counts as plain mode."""
     roots = [69300, 69615, 69700, 69905]  # 69615 = 3^2 * 5 * 7 * 13 * 17
     cw, lw, pw, hw, sw = hc_search(hc, 70000, 70000, roots)
-    hc.hc_set_tuning(0xFFFFFFFF, 0.0, 8)
+    hc.hc_set_tuning(0.0, 0, 8)
     cp, lp, pp, hp, sp = hc_search(hc, 70000, 70000, roots)
-    hc.hc_set_tuning(1, 1.0, 8)
+    hc.hc_set_tuning(1.0, 0, 8)
     assert (cw, lw, pw, hw) == (cp, lp, pp, hp)
     assert sw.window_tiles > 0
     assert sw.cells < 0.5 * sp.cells, (sw.cells, sp.cells)
