@@ -60,16 +60,27 @@ struct Counters {
 // Everything one launch of the search needs besides the resident tables.  A
 // buffer set is reused only after its launch has completed.
 struct LaunchBuffers {
-    DevBuf pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
+    DevBuf header, pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
     HostBuf upload;   // pinned staging of the plan
     HostBuf h_misc;   // pinned copy of misc: [1] deferred, [2] raw, [3] hits, [6..9] evaluated (2 x u64)
     uint32_t def_cap = 0;
-    hipEvent_t ev[5] = {};  // start, setup done, window cells done, plain cells done, end
+    static constexpr int STAGES = 4;
+    // start, setup done, window cells done, plain cells done, exact done, misc read back
+    hipEvent_t ev[6] = {};
     Plan plan;
     uint32_t npos = 0;
+    // The kernel chain (left-set sort and build with the counter reset; the
+    // window cell kernel; the plain cell kernel; the exact and acceptance
+    // kernels) captured as STAGES graphs; valid while the buffers' addresses
+    // and def_cap match `layout`.
+    hipGraph_t graph[STAGES] = {};
+    hipGraphExec_t exec[STAGES] = {};
+    std::vector<uintptr_t> layout;
+    bool replayed = false;  // the last launch replayed the graph
     static constexpr uint32_t RAW_CAP = 1u << 16, HIT_CAP = 1u << 16;
     hipError_t create();
     void release();
+    void drop_graph();
 };
 
 // Outcome of a completed launch.
@@ -78,6 +89,7 @@ struct LaunchResult {
     uint32_t deferred = 0;
     uint32_t raw = 0;
     std::vector<HitOut> hits;  // accepted (not de-duplicated), HitOut::pos = plan position
+    bool graph = false;        // the kernel chain ran as a graph replay
     uint64_t evaluated = 0;    // leaves evaluated by the fast path
     uint64_t win_evaluated = 0, plain_evaluated = 0;
     float setup_ms = 0, cells_ms = 0, exact_ms = 0;
@@ -113,6 +125,9 @@ struct hss_ctx {
     hss_stats stats{};
     hss::Tuning tune{};
     int win_blocks_per_cu = 1;
+    bool use_graphs = true;            // capture the kernel chain once, replay it per batch
+    std::string graph_error;           // why capture was given up, if it was
+    hipStream_t capture = nullptr;     // stream the chain is captured on (never executes work)
     hss_server* server = nullptr;  // a server drives the context while attached
 
     uint64_t waves() const { return (uint64_t)cus * blocks_per_cu * 4; }
@@ -128,7 +143,7 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n);
 // the positions' counters, cells, exact and acceptance kernels, read-back of
 // the misc counters into pinned memory.  Asynchronous.
 int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
-// After B.ev[4] has completed: read the misc counters, detect overflow, copy
+// After B.ev[5] has completed: read the misc counters, detect overflow, copy
 // accepted hits on `copy` (a stream with no work that writes B's buffers).
 int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out);
 

@@ -24,10 +24,9 @@ struct CellParams {
     // batch
     const PosDesc* pos;
     const LeftDesc* lefts;
-    const TileDesc* tiles;    // plain mode
-    uint32_t ntiles;
-    const TileDesc* wtiles;   // window mode
-    uint32_t nwtiles;
+    const BatchHeader* hdr;   // batch sizes (device memory)
+    const TileDesc* tiles;    // plain mode, hdr->ntiles
+    const TileDesc* wtiles;   // window mode, hdr->nwtiles
     const uint32_t* btab;     // bucket tables of the sorted left sets, WIN_NB + 1 each
     const LeftFast* lfast;
     const i128* lre;
@@ -56,16 +55,17 @@ hipError_t launch_prime_values(const uint32_t* plist, uint32_t np, const uint32_
 hipError_t launch_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t n, uint32_t total,
                              const uint32_t* vbase, const i128* vre, const i128* vim, double* cre, double* cim,
                              uint32_t* cres, hipStream_t st);
-hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uint32_t nleft, const uint32_t* vbase,
-                             const i128* vre, const i128* vim, const uint32_t* sorted, uint32_t nsorted,
+// Fixed-grid launches (sizes from hdr): capturable once and replayable.
+hipError_t launch_build_left(const BatchHeader* hdr, const LeftBuild* builds, const LeftDesc* lefts,
+                             const uint32_t* vbase, const i128* vre, const i128* vim, const uint32_t* sorted,
                              uint32_t* perm, uint32_t* btab, i128* lre, i128* lim, LeftFast* lfast, hipStream_t st);
 hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st);
 hipError_t launch_cells_win(const CellParams& P, uint32_t blocks, hipStream_t st);
 hipError_t launch_exact(const CellParams& P, uint32_t blocks, hipStream_t st);
 hipError_t launch_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* nraw, uint32_t raw_cap, HitOut* hits,
                          uint32_t* nhits, uint32_t hit_cap, hipStream_t st);
-hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand, unsigned long long* leaves,
-                              unsigned long long* pairs, uint32_t* misc, hipStream_t st);
+hipError_t launch_batch_reset(const BatchHeader* hdr, const PosDesc* pos, unsigned long long* cand,
+                              unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc, hipStream_t st);
 hipError_t cell_occupancy(int* blocks_per_cu);
 hipError_t cell_win_occupancy(int* blocks_per_cu);
 

@@ -234,87 +234,96 @@ __device__ inline GInt left_value(const LeftBuild& b, uint32_t t, const uint32_t
     return z;
 }
 
-// One block per left set.  Entry t0 holds rank perm[off + t0] for a sorted
-// left set (window mode), rank t0 otherwise.
-__global__ __launch_bounds__(256) void k_build_left(const LeftBuild* builds, const LeftDesc* lefts, const uint32_t* vbase,
-                             const i128* vre, const i128* vim, const uint32_t* perm, i128* lre, i128* lim,
-                             LeftFast* lfast) {
-    const LeftBuild b = builds[blockIdx.x];
-    const LeftDesc L = lefts[blockIdx.x];
-    for (uint32_t t0 = threadIdx.x; t0 < L.H; t0 += blockDim.x) {
-        const uint32_t e = L.off + t0;
-        const uint32_t rank = L.bt != NO_TABLE ? perm[e] : t0;
-        GInt z = left_value(b, rank, vbase, vre, vim);
-        lre[e] = z.re;
-        lim[e] = z.im;
-        LeftFast f;
-        f.re = i128_to_double(z.re);
-        f.im = i128_to_double(z.im);
-        f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
-        f.pad = 0;
-        lfast[e] = f;
+// One block per left set (grid-stride over hdr->nleft).  Entry t0 holds rank
+// perm[off + t0] for a sorted left set (window mode), rank t0 otherwise.
+__global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, const LeftBuild* builds,
+                                                    const LeftDesc* lefts, const uint32_t* vbase, const i128* vre,
+                                                    const i128* vim, const uint32_t* perm, i128* lre, i128* lim,
+                                                    LeftFast* lfast) {
+    const uint32_t nleft = hdr->nleft;
+    for (uint32_t li = blockIdx.x; li < nleft; li += gridDim.x) {
+        const LeftBuild b = builds[li];
+        const LeftDesc L = lefts[li];
+        for (uint32_t t0 = threadIdx.x; t0 < L.H; t0 += blockDim.x) {
+            const uint32_t e = L.off + t0;
+            const uint32_t rank = L.bt != NO_TABLE ? perm[e] : t0;
+            GInt z = left_value(b, rank, vbase, vre, vim);
+            lre[e] = z.re;
+            lim[e] = z.im;
+            LeftFast f;
+            f.re = i128_to_double(z.re);
+            f.im = i128_to_double(z.im);
+            f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
+            f.pad = 0;
+            lfast[e] = f;
+        }
     }
 }
 
-// One block per sorted left set (index i of the plan's sorted list): order
+// One block per sorted left set (grid-stride over hdr->nsorted): order
 // the non-central entries by psi (angle modulo pi/2, from the same FP64
 // coordinates k_build_left stores), write the permutation and the bucket
 // table bt[b] = number of entries with psi_bucket < b, b = 0..WIN_NB.
-__global__ __launch_bounds__(256) void k_left_sort(const uint32_t* sorted, const LeftBuild* builds,
-                                                   const LeftDesc* lefts, const uint32_t* vbase, const i128* vre,
-                                                   const i128* vim, uint32_t* perm, uint32_t* btab) {
+__global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const uint32_t* sorted,
+                                                   const LeftBuild* builds, const LeftDesc* lefts,
+                                                   const uint32_t* vbase, const i128* vre, const i128* vim,
+                                                   uint32_t* perm, uint32_t* btab) {
     __shared__ double key[WIN_MAX_SORT];
     __shared__ uint16_t idx[WIN_MAX_SORT];
-    const uint32_t li = sorted[blockIdx.x];
-    const LeftBuild b = builds[li];
-    const LeftDesc L = lefts[li];
-    const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
-    uint32_t n2 = 1;
-    while (n2 < n) n2 <<= 1;
-    for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) {
-        double k = 1e300;
-        if (t < n) {
-            GInt z = left_value(b, t, vbase, vre, vim);
-            k = psi_of(i128_to_double(z.re), i128_to_double(z.im));
-        }
-        key[t] = k;
-        idx[t] = (uint16_t)t;
-    }
-    __syncthreads();
-    // bitonic sort of (key, idx) over n2 entries
-    for (uint32_t size = 2; size <= n2; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
-                const uint32_t i = 2 * t - (t & (stride - 1));
-                const uint32_t j = i + stride;
-                const bool up = (i & size) == 0;
-                const double ki = key[i], kj = key[j];
-                const bool sw = up ? (ki > kj || (ki == kj && idx[i] > idx[j]))
-                                   : (ki < kj || (ki == kj && idx[i] < idx[j]));
-                if (sw) {
-                    key[i] = kj;
-                    key[j] = ki;
-                    const uint16_t ti = idx[i];
-                    idx[i] = idx[j];
-                    idx[j] = ti;
-                }
+    const uint32_t nsorted = hdr->nsorted;
+    for (uint32_t bi = blockIdx.x; bi < nsorted; bi += gridDim.x) {  // uniform per block
+        const uint32_t li = sorted[bi];
+        const LeftBuild b = builds[li];
+        const LeftDesc L = lefts[li];
+        const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
+        uint32_t n2 = 1;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) {
+            double k = 1e300;
+            if (t < n) {
+                GInt z = left_value(b, t, vbase, vre, vim);
+                k = psi_of(i128_to_double(z.re), i128_to_double(z.im));
             }
-            __syncthreads();
+            key[t] = k;
+            idx[t] = (uint16_t)t;
         }
-    }
-    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) perm[L.off + t] = idx[t];
-    if (threadIdx.x == 0) perm[L.off + n] = n;  // the central entry stays last
-    uint32_t* bt = btab + (size_t)L.bt * (WIN_NB + 1);
-    for (uint32_t bk = threadIdx.x; bk <= WIN_NB; bk += blockDim.x) {
-        uint32_t lo = 0, hi = n;  // first t with psi_bucket(key[t]) >= bk
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (psi_bucket<WIN_NB>(key[mid]) < bk)
-                lo = mid + 1;
-            else
-                hi = mid;
+        __syncthreads();
+        // bitonic sort of (key, idx) over n2 entries
+        for (uint32_t size = 2; size <= n2; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
+                    const uint32_t i = 2 * t - (t & (stride - 1));
+                    const uint32_t j = i + stride;
+                    const bool up = (i & size) == 0;
+                    const double ki = key[i], kj = key[j];
+                    const bool sw = up ? (ki > kj || (ki == kj && idx[i] > idx[j]))
+                                       : (ki < kj || (ki == kj && idx[i] < idx[j]));
+                    if (sw) {
+                        key[i] = kj;
+                        key[j] = ki;
+                        const uint16_t ti = idx[i];
+                        idx[i] = idx[j];
+                        idx[j] = ti;
+                    }
+                }
+                __syncthreads();
+            }
         }
-        bt[bk] = bk == WIN_NB ? n : lo;
+        for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) perm[L.off + t] = idx[t];
+        if (threadIdx.x == 0) perm[L.off + n] = n;  // the central entry stays last
+        uint32_t* bt = btab + (size_t)L.bt * (WIN_NB + 1);
+        for (uint32_t bk = threadIdx.x; bk <= WIN_NB; bk += blockDim.x) {
+            uint32_t lo = 0, hi = n;  // first t with psi_bucket(key[t]) >= bk
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (psi_bucket<WIN_NB>(key[mid]) < bk)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            bt[bk] = bk == WIN_NB ? n : lo;
+        }
+        __syncthreads();  // key[] is rewritten by the next left set
     }
 }
 
@@ -441,7 +450,7 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
     for (;;) {
         // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_tile, lane == 0 ? 1u : 0u));
-        if (t >= P.ntiles) break;
+        if (t >= P.hdr->ntiles) break;
         const TileDesc T = P.tiles[t];
         const LeftDesc L = P.lefts[T.left];
         const PosDesc& S = P.pos[L.pos];
@@ -563,7 +572,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
     WinRec* rec = s_rec[wave];
     for (;;) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_wtile, lane == 0 ? 1u : 0u));
-        if (t >= P.nwtiles) break;
+        if (t >= P.hdr->nwtiles) break;
         const TileDesc T = P.wtiles[t];
         const LeftDesc L = P.lefts[T.left];
         const PosDesc& S = P.pos[L.pos];
@@ -772,8 +781,9 @@ __global__ __launch_bounds__(256) void k_accept(const PosDesc* pos, const HitOut
 // Start of a batch: zero the work-queue heads and the deferred / raw / hit
 // counters (misc[0..15]) and the counters of the batch's positions.  A kernel
 // rather than memset nodes, so the launch sequence can be captured in a graph.
-__global__ void k_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand,
+__global__ void k_batch_reset(const BatchHeader* hdr, const PosDesc* pos, unsigned long long* cand,
                               unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc) {
+    const uint32_t npos = hdr->npos;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 16) misc[i] = 0;
     for (; i < npos; i += gridDim.x * blockDim.x) {
@@ -838,14 +848,12 @@ hipError_t launch_fill_cache(const uint32_t* spf, const uint32_t* off, uint64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_build_left(const LeftBuild* builds, const LeftDesc* lefts, uint32_t nleft, const uint32_t* vbase,
-                             const i128* vre, const i128* vim, const uint32_t* sorted, uint32_t nsorted,
+hipError_t launch_build_left(const BatchHeader* hdr, const LeftBuild* builds, const LeftDesc* lefts,
+                             const uint32_t* vbase, const i128* vre, const i128* vim, const uint32_t* sorted,
                              uint32_t* perm, uint32_t* btab, i128* lre, i128* lim, LeftFast* lfast, hipStream_t st) {
-    if (!nleft) return hipSuccess;
-    if (nsorted)
-        hipLaunchKernelGGL(k_left_sort, dim3(nsorted), dim3(256), 0, st, sorted, builds, lefts, vbase, vre, vim, perm,
-                           btab);
-    hipLaunchKernelGGL(k_build_left, dim3(nleft), dim3(256), 0, st, builds, lefts, vbase, vre, vim,
+    hipLaunchKernelGGL(k_left_sort, dim3(1024), dim3(256), 0, st, hdr, sorted, builds, lefts, vbase, vre, vim, perm,
+                       btab);
+    hipLaunchKernelGGL(k_build_left, dim3(2048), dim3(256), 0, st, hdr, builds, lefts, vbase, vre, vim,
                        (const uint32_t*)perm, lre, lim, lfast);
     return hipGetLastError();
 }
@@ -871,11 +879,9 @@ hipError_t launch_accept(const PosDesc* pos, const HitOut* raw, const uint32_t* 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_reset(const PosDesc* pos, uint32_t npos, unsigned long long* cand, unsigned long long* leaves,
-                              unsigned long long* pairs, uint32_t* misc, hipStream_t st) {
-    uint32_t blocks = (npos + 255) / 256;
-    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);  // >= 1 block: misc is always cleared
-    hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, st, pos, npos, cand, leaves, pairs, misc);
+hipError_t launch_batch_reset(const BatchHeader* hdr, const PosDesc* pos, unsigned long long* cand,
+                              unsigned long long* leaves, unsigned long long* pairs, uint32_t* misc, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(256), dim3(256), 0, st, hdr, pos, cand, leaves, pairs, misc);
     return hipGetLastError();
 }
 

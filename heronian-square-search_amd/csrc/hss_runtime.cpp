@@ -72,9 +72,20 @@ hipError_t LaunchBuffers::create() {
     return h_misc.reserve(64);
 }
 
+void LaunchBuffers::drop_graph() {
+    for (int i = 0; i < STAGES; ++i) {
+        if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+        if (graph[i]) (void)hipGraphDestroy(graph[i]);
+        exec[i] = nullptr;
+        graph[i] = nullptr;
+    }
+    layout.clear();
+}
+
 void LaunchBuffers::release() {
-    for (DevBuf* b : {&pos, &lefts, &builds, &tiles, &wtiles, &sorted, &perm, &btab, &lfast, &lre, &lim, &misc, &def,
-                      &raw, &hits})
+    drop_graph();
+    for (DevBuf* b : {&header, &pos, &lefts, &builds, &tiles, &wtiles, &sorted, &perm, &btab, &lfast, &lre, &lim, &misc,
+                      &def, &raw, &hits})
         b->release();
     upload.release();
     h_misc.release();
@@ -132,6 +143,7 @@ extern "C" void hss_ctx_destroy(hss_ctx* c) {
     (void)hipFree(c->d_vim);
     c->sync.release();
     c->counters.release();
+    if (c->capture) (void)hipStreamDestroy(c->capture);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -249,8 +261,10 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(HSS_EDEVICE, "device properties"));
     c->cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(HSS_EDEVICE, "stream create failed"));
+    c->use_graphs = std::getenv("HSS_NO_GRAPH") == nullptr && !verbose();
     if (c->sync.create() != hipSuccess) return bail(fail(HSS_EDEVICE, "launch buffers: event or pinned memory"));
     int bpc = 0;
     if (cell_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
@@ -264,6 +278,14 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
     if (rc != HSS_OK) return bail(rc);
     *out = c;
     return HSS_OK;
+}
+
+extern "C" int hss_ctx_use_graphs(hss_ctx* c, int enable) {
+    if (!c) return fail(HSS_EINVAL, "null context");
+    const int old = c->use_graphs ? 1 : 0;
+    c->use_graphs = enable != 0;
+    if (!enable) c->sync.drop_graph();
+    return old;
 }
 
 extern "C" int hss_ctx_get_tuning(const hss_ctx* c, hss_tuning* out) {
@@ -398,11 +420,73 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
     return HSS_OK;
 }
 
+// The kernel chain of one launch in four stages on stream st (timing events
+// go between the stages): 0 = left-set sort and build and the counter reset,
+// 1 = window-mode cell kernel, 2 = plain cell kernel, 3 = exact and
+// acceptance kernels.  Every size comes from the device header and every grid
+// is fixed, so one capture of a stage serves every batch.
+static int enqueue_stage(hss_ctx* c, LaunchBuffers& B, const CellParams& K, int stage, hipStream_t st) {
+    uint32_t* misc = B.misc.as<uint32_t>();
+    if (stage == 0) {
+        HIP_TRY(launch_build_left(B.header.as<BatchHeader>(), B.builds.as<LeftBuild>(), B.lefts.as<LeftDesc>(),
+                                  c->d_vbase, c->d_vre, c->d_vim, B.sorted.as<uint32_t>(), B.perm.as<uint32_t>(),
+                                  B.btab.as<uint32_t>(), B.lre.as<i128>(), B.lim.as<i128>(), B.lfast.as<LeftFast>(),
+                                  st));
+        HIP_TRY(launch_batch_reset(B.header.as<BatchHeader>(), K.pos, K.cand, K.leaves, K.pairs, misc, st));
+    } else if (stage == 1) {
+        HIP_TRY(launch_cells_win(K, c->win_grid_blocks(), st));
+    } else if (stage == 2) {
+        HIP_TRY(launch_cells(K, c->grid_blocks(), st));
+    } else {
+        HIP_TRY(launch_exact(K, (uint32_t)c->cus * 4, st));
+        HIP_TRY(launch_accept(K.pos, K.raw, K.nraw, LaunchBuffers::RAW_CAP, B.hits.as<HitOut>(), misc + 3,
+                              LaunchBuffers::HIT_CAP, st));
+    }
+    return HSS_OK;
+}
+
+// Capture the stages (once per buffer layout).  false: capture is not
+// available here (the reason goes to c->graph_error).
+static bool capture_stages(hss_ctx* c, LaunchBuffers& B, const CellParams& K) {
+    B.drop_graph();
+    std::string why;
+    for (int stage = 0; stage < LaunchBuffers::STAGES; ++stage) {
+        hipError_t e = hipStreamBeginCapture(c->capture, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            why = hipGetErrorString(e);
+            break;
+        }
+        const int rc = enqueue_stage(c, B, K, stage, c->capture);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(c->capture, &g);
+        if (rc != HSS_OK || e != hipSuccess || !g) {
+            why = rc != HSS_OK ? std::string(hss_last_error()) : std::string(hipGetErrorString(e));
+            if (g) (void)hipGraphDestroy(g);
+            break;
+        }
+        e = hipGraphInstantiate(&B.exec[stage], g, nullptr, nullptr, 0);
+        B.graph[stage] = g;
+        if (e != hipSuccess) {
+            B.exec[stage] = nullptr;
+            why = hipGetErrorString(e);
+            break;
+        }
+    }
+    if (!why.empty()) {
+        B.drop_graph();
+        c->graph_error = why;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
 int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
     const Plan& P = B.plan;
     const uint32_t npos = B.npos, nleft = (uint32_t)P.lefts.size(), ntiles = (uint32_t)P.tiles.size();
     const uint32_t nwtiles = (uint32_t)P.wtiles.size(), nsorted = (uint32_t)P.sorted.size();
     const size_t entries = std::max<uint64_t>(P.left_entries, 1);
+    HIP_TRY(B.header.reserve(sizeof(BatchHeader)));
     HIP_TRY(B.pos.reserve(std::max<size_t>(npos, 1) * sizeof(PosDesc)));
     HIP_TRY(B.lefts.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftDesc)));
     HIP_TRY(B.builds.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftBuild)));
@@ -421,19 +505,27 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
         B.def_cap = 1u << 22;
         HIP_TRY(B.def.reserve((size_t)B.def_cap * sizeof(DefItem)));
     }
-    // stage the plan in pinned memory: one region per array, 256-byte aligned
+    // stage the header and the plan in pinned memory: one region per array, 256-byte aligned
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t b_pos = al(npos * sizeof(PosDesc)), b_left = al(nleft * sizeof(LeftDesc)),
-                 b_build = al(nleft * sizeof(LeftBuild)), b_tile = al(ntiles * sizeof(TileDesc)),
-                 b_wtile = al(nwtiles * sizeof(TileDesc)), b_sorted = al(nsorted * 4);
-    HIP_TRY(B.upload.reserve(b_pos + b_left + b_build + b_tile + b_wtile + b_sorted));
-    char* up = B.upload.as<char>();
-    char* const up_pos = up;
+    const size_t b_hdr = al(sizeof(BatchHeader)), b_pos = al(npos * sizeof(PosDesc)),
+                 b_left = al(nleft * sizeof(LeftDesc)), b_build = al(nleft * sizeof(LeftBuild)),
+                 b_tile = al(ntiles * sizeof(TileDesc)), b_wtile = al(nwtiles * sizeof(TileDesc)),
+                 b_sorted = al(nsorted * 4);
+    HIP_TRY(B.upload.reserve(b_hdr + b_pos + b_left + b_build + b_tile + b_wtile + b_sorted));
+    char* const up_hdr = B.upload.as<char>();
+    char* const up_pos = up_hdr + b_hdr;
     char* const up_left = up_pos + b_pos;
     char* const up_build = up_left + b_left;
     char* const up_tile = up_build + b_build;
     char* const up_wtile = up_tile + b_tile;
     char* const up_sorted = up_wtile + b_wtile;
+    BatchHeader hdr{};
+    hdr.npos = npos;
+    hdr.nleft = nleft;
+    hdr.nsorted = nsorted;
+    hdr.ntiles = ntiles;
+    hdr.nwtiles = nwtiles;
+    std::memcpy(up_hdr, &hdr, sizeof hdr);
     std::memcpy(up_pos, P.pos.data(), npos * sizeof(PosDesc));
     std::memcpy(up_left, P.lefts.data(), nleft * sizeof(LeftDesc));
     std::memcpy(up_build, P.builds.data(), nleft * sizeof(LeftBuild));
@@ -442,6 +534,7 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     std::memcpy(up_sorted, P.sorted.data(), nsorted * 4);
 
     HIP_TRY(hipEventRecord(B.ev[0], st));
+    HIP_TRY(hipMemcpyAsync(B.header.p, up_hdr, sizeof hdr, hipMemcpyHostToDevice, st));
     if (npos) HIP_TRY(hipMemcpyAsync(B.pos.p, up_pos, npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
     if (nleft) {
         HIP_TRY(hipMemcpyAsync(B.lefts.p, up_left, nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
@@ -451,13 +544,9 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     if (nwtiles)
         HIP_TRY(hipMemcpyAsync(B.wtiles.p, up_wtile, nwtiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
     if (nsorted) HIP_TRY(hipMemcpyAsync(B.sorted.p, up_sorted, nsorted * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_build_left(B.builds.as<LeftBuild>(), B.lefts.as<LeftDesc>(), nleft, c->d_vbase, c->d_vre,
-                              c->d_vim, B.sorted.as<uint32_t>(), nsorted, B.perm.as<uint32_t>(),
-                              B.btab.as<uint32_t>(), B.lre.as<i128>(), B.lim.as<i128>(), B.lfast.as<LeftFast>(),
-                              st));
-    // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile
+
+    // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile, [6..9] evaluated
     uint32_t* misc = B.misc.as<uint32_t>();
-    HIP_TRY(launch_batch_reset(B.pos.as<PosDesc>(), npos, cnt.cand, cnt.leaves, cnt.pairs, misc, st));
     CellParams K{};
     K.off = c->d_off;
     K.cre = c->d_cre;
@@ -465,10 +554,9 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.cres = c->d_cres;
     K.pos = B.pos.as<PosDesc>();
     K.lefts = B.lefts.as<LeftDesc>();
+    K.hdr = B.header.as<BatchHeader>();
     K.tiles = B.tiles.as<TileDesc>();
-    K.ntiles = ntiles;
     K.wtiles = B.wtiles.as<TileDesc>();
-    K.nwtiles = nwtiles;
     K.btab = B.btab.as<uint32_t>();
     K.lfast = B.lfast.as<LeftFast>();
     K.lre = B.lre.as<i128>();
@@ -486,20 +574,39 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.nraw = misc + 2;
     K.raw_cap = LaunchBuffers::RAW_CAP;
     K.qr = c->qr;
-    HIP_TRY(hipEventRecord(B.ev[1], st));
-    vlog(st, "left sets built");
-    if (nwtiles) HIP_TRY(launch_cells_win(K, c->win_grid_blocks(), st));
-    vlog(st, "window cell kernel");
-    HIP_TRY(hipEventRecord(B.ev[2], st));
-    if (ntiles) HIP_TRY(launch_cells(K, c->grid_blocks(), st));
-    vlog(st, "cell kernel");
-    HIP_TRY(hipEventRecord(B.ev[3], st));
-    HIP_TRY(launch_exact(K, (uint32_t)c->cus * 4, st));
-    HIP_TRY(launch_accept(K.pos, K.raw, K.nraw, LaunchBuffers::RAW_CAP, B.hits.as<HitOut>(), misc + 3,
-                          LaunchBuffers::HIT_CAP, st));
-    vlog(st, "exact + accept");
+
+    bool replay = false;
+    if (c->use_graphs) {
+        std::vector<uintptr_t> layout;
+        for (const DevBuf* d : {&B.header, &B.pos, &B.lefts, &B.builds, &B.tiles, &B.wtiles, &B.sorted, &B.perm,
+                                &B.btab, &B.lfast, &B.lre, &B.lim, &B.misc, &B.def, &B.raw, &B.hits})
+            layout.push_back((uintptr_t)d->p);
+        for (const void* p : {(const void*)cnt.cand, (const void*)cnt.leaves, (const void*)cnt.pairs})
+            layout.push_back((uintptr_t)p);
+        layout.push_back(B.def_cap);
+        if (!B.exec[0] || layout != B.layout) {
+            if (capture_stages(c, B, K))
+                B.layout = layout;
+            else
+                c->use_graphs = false;  // launch the stages directly from now on
+        }
+        replay = B.exec[0] != nullptr;
+    }
+    for (int stage = 0; stage < LaunchBuffers::STAGES; ++stage) {
+        const bool empty = (stage == 1 && !nwtiles) || (stage == 2 && !ntiles);  // nothing to run
+        if (empty) {
+        } else if (replay) {
+            HIP_TRY(hipGraphLaunch(B.exec[stage], st));
+        } else {
+            const int rc = enqueue_stage(c, B, K, stage, st);
+            if (rc != HSS_OK) return rc;
+        }
+        HIP_TRY(hipEventRecord(B.ev[1 + stage], st));
+    }
+    B.replayed = replay;
+    vlog(st, "kernel chain");
     HIP_TRY(hipMemcpyAsync(B.h_misc.p, misc, 48, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipEventRecord(B.ev[4], st));
+    HIP_TRY(hipEventRecord(B.ev[5], st));
     return HSS_OK;
 }
 
@@ -511,6 +618,7 @@ int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult&
     std::memcpy(&out.win_evaluated, m4 + 6, 8);
     std::memcpy(&out.plain_evaluated, m4 + 8, 8);
     out.evaluated = out.win_evaluated + out.plain_evaluated;
+    out.graph = B.replayed;
     (void)hipEventElapsedTime(&out.setup_ms, B.ev[0], B.ev[1]);
     (void)hipEventElapsedTime(&out.win_ms, B.ev[1], B.ev[2]);
     (void)hipEventElapsedTime(&out.plain_ms, B.ev[2], B.ev[3]);
@@ -565,7 +673,7 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     for (int attempt = 0;; ++attempt) {
         rc = launch_enqueue(c, B, cnt, c->stream);
         if (rc != HSS_OK) return rc;
-        HIP_TRY(hipEventSynchronize(B.ev[4]));
+        HIP_TRY(hipEventSynchronize(B.ev[5]));
         rc = launch_collect(c, B, c->stream, R);
         if (rc != HSS_OK) return rc;
         if (!R.rerun) break;
@@ -586,6 +694,7 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     c->stats.tiles = B.plan.tiles.size() + B.plan.wtiles.size();
     c->stats.window_tiles = B.plan.wtiles.size();
     c->stats.evaluated = R.evaluated;
+    c->stats.graph_launch = R.graph ? 1 : 0;
     c->stats.window_evaluated = R.win_evaluated;
     c->stats.window_ms = R.win_ms;
     c->stats.plain_ms = R.plain_ms;

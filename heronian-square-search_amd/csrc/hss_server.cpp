@@ -237,7 +237,7 @@ void worker_main(hss_server* S) {
         inflight.pop_front();
         LaunchBuffers& B = S->bufs[L.buf];
         LaunchResult R;
-        int rc = hipEventSynchronize(B.ev[4]) == hipSuccess ? HSS_OK : fail(HSS_EDEVICE, "launch failed");
+        int rc = hipEventSynchronize(B.ev[5]) == hipSuccess ? HSS_OK : fail(HSS_EDEVICE, "launch failed");
         if (rc == HSS_OK) rc = launch_collect(c, B, S->copy, R);
         if (rc == HSS_OK && R.rerun) {
             if (++L.attempts > 3) {

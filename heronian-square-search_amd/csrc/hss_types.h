@@ -69,6 +69,14 @@ struct LeftFast {
     uint32_t pad;
 };
 
+// Sizes of one batch, read by the kernels from device memory so that the
+// launch sequence (fixed grids) can be captured once as a graph and replayed
+// for batches of any size.
+struct BatchHeader {
+    uint32_t npos, nleft, nsorted, ntiles, nwtiles;
+    uint32_t pad[3];
+};
+
 // A leaf sent to the exact kernel.  kflags = k | conj << 30 | full << 31, where
 // full = 1 means the fast path left the leaf's count undecided.
 struct DefItem {

@@ -42,7 +42,7 @@ class Stats(ctypes.Structure):
                 ("block_threads", ctypes.c_uint32), ("window_tiles", ctypes.c_uint64),
                 ("sorted_lefts", ctypes.c_uint64), ("evaluated", ctypes.c_uint64), ("window_evaluated", ctypes.c_uint64),
                 ("window_ms", ctypes.c_double),
-                ("plain_ms", ctypes.c_double)]
+                ("plain_ms", ctypes.c_double), ("graph_launch", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -119,6 +119,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.hss_search_interval.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(Hit), u32,
                                         ctypes.POINTER(u32)]
     lib.hss_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.hss_ctx_use_graphs.argtypes = [vp, ctypes.c_int]
     lib.hss_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     lib.hss_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     lib.hss_autotune.argtypes = [vp, u64, ctypes.POINTER(u64), u32, u32, ctypes.POINTER(Tuning),
@@ -205,6 +206,11 @@ class Context:
         s = Stats()
         _check(_lib.hss_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
+
+    def use_graphs(self, enable: bool) -> bool:
+        """Replay the kernel chain as a captured hipGraph (default) or launch it
+        directly; returns the previous setting."""
+        return bool(_lib.hss_ctx_use_graphs(self._h, 1 if enable else 0))
 
     def get_tuning(self) -> dict:
         t = Tuning()

@@ -69,6 +69,7 @@ typedef struct {
     uint64_t window_evaluated; /* ... of them by the window-mode kernel */
     double window_ms;      /* the window-mode kernel's part of cells_ms */
     double plain_ms;       /* the plain kernel's part of cells_ms */
+    uint64_t graph_launch; /* 1 when the kernel chain ran as a replay of its captured hipGraph */
 } hss_stats;
 
 /*
@@ -133,6 +134,15 @@ typedef struct {
     double win_item_cost;    /* planning cost of a window-mode leaf vs a plain leaf */
     double win_partner_cost; /* planning cost per partner entry of a window-mode tile */
 } hss_tuning;
+
+/*
+ * Graph replay of the kernel chain (left-set sort and build, counter reset,
+ * window and plain cell kernels, exact and acceptance kernels): captured once
+ * per buffer layout and replayed for every batch, sizes read from a device
+ * header (enable = 1, the default; HSS_NO_GRAPH in the environment starts
+ * with 0).  Returns the previous setting.
+ */
+int hss_ctx_use_graphs(hss_ctx* ctx, int enable);
 
 int hss_ctx_get_tuning(const hss_ctx* ctx, hss_tuning* out);
 int hss_ctx_set_tuning(hss_ctx* ctx, const hss_tuning* in);
