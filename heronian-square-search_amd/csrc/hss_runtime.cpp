@@ -594,10 +594,8 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     }
     for (int stage = 0; stage < LaunchBuffers::STAGES; ++stage) {
         const bool empty = (stage == 1 && !nwtiles) || (stage == 2 && !ntiles);  // nothing to run
-        if (empty) {
-        } else if (replay) {
-            HIP_TRY(hipGraphLaunch(B.exec[stage], st));
-        } else {
+        if (!empty && replay) HIP_TRY(hipGraphLaunch(B.exec[stage], st));
+        if (!empty && !replay) {
             const int rc = enqueue_stage(c, B, K, stage, st);
             if (rc != HSS_OK) return rc;
         }
