@@ -103,6 +103,15 @@ uint64_t left_size(const RootFactors& f, const uint32_t* sexp) {
 
 }  // namespace
 
+uint32_t gaussian_weight(uint64_t l, const std::vector<uint32_t>& primes) {
+    if (!l) return 0;
+    RootFactors f;
+    if (!factor_root(l, primes, f)) return 0;
+    uint32_t w = 1;
+    for (uint32_t j = 0; j < f.ns; ++j) w *= 4 * f.se[j] + 1;
+    return w;
+}
+
 double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>& small_primes, const uint32_t* off,
                          uint64_t n_max) {
     if (limit < 1 || limit > n_max || l < 1 || l > limit) return -1.0;

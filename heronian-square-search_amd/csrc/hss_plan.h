@@ -84,6 +84,11 @@ bool plan_batch(const PosReq* reqs, uint32_t n, const std::vector<uint32_t>& sma
 // limit (sampled at a few k).
 double tile_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1, int samples);
 
+// Gaussian weight w(l) = prod (4e + 1) over the prime powers p^e || l with
+// p = 1 (mod 4) (experiments/heavy/README.md:9); 0 if l cannot be factored by
+// `primes` (needs every prime <= sqrt(l)).
+uint32_t gaussian_weight(uint64_t l, const std::vector<uint32_t>& primes);
+
 // Work estimate of one root position in cells (left entry x partner half-set
 // entry), the same measure plan_batch sizes tiles with; -1 if l cannot be
 // planned (l outside [1, limit], limit > n_max, too many distinct primes).

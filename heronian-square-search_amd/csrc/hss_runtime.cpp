@@ -730,6 +730,12 @@ extern "C" int hss_search_interval(hss_ctx* c, uint64_t limit, uint64_t first, u
                       n_hits);
 }
 
+extern "C" uint32_t hss_gaussian_weight(uint64_t l) {
+    static const std::vector<uint32_t> primes = primes_upto(1u << 13);  // covers l <= 2^26
+    if (!l || l > (1ull << 26)) return 0;
+    return gaussian_weight(l, primes);
+}
+
 extern "C" int hss_verify_triple(uint64_t l, uint64_t m, uint64_t n, uint64_t area_lo, uint64_t area_hi) {
     u128 area = ((u128)area_hi << 64) | area_lo;
     if (area >> 124) return 0;

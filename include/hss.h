@@ -171,6 +171,14 @@ int hss_get_stats(const hss_ctx* ctx, hss_stats* out);
 int hss_verify_triple(uint64_t l, uint64_t m, uint64_t n, uint64_t area_lo, uint64_t area_hi);
 
 /*
+ * Gaussian weight of a root side: w(l) = prod (4e + 1) over the prime powers
+ * p^e || l with p = 1 (mod 4) (experiments/heavy/README.md:9; branch_weight of
+ * experiments/heavy/gpu_search_v36.cu:51-55).  Host arithmetic, l <= 2^26;
+ * 0 for l = 0 or l > 2^26.
+ */
+uint32_t hss_gaussian_weight(uint64_t l);
+
+/*
  * Work estimate of the root positions [first, last] at `limit`: the planner's
  * count of (left entry, partner half-set entry) cells, the unit the server's
  * launch budget and the tile sizes are measured in.  Host arithmetic only.

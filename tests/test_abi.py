@@ -92,3 +92,36 @@ def test_verify_triple_matches_results_files():
     assert lines
     for l, m, n, a in lines:
         assert hss.verify_triple(l, m, n, a)
+
+
+def _weight(x: int) -> int:
+    """Gaussian weight by trial division (experiments/heavy/README.md:9)."""
+    r, p = 1, 2
+    while p * p <= x:
+        e = 0
+        while x % p == 0:
+            x //= p
+            e += 1
+        if p % 4 == 1:
+            r *= 4 * e + 1
+        p += 1
+    if x > 1 and x % 4 == 1:
+        r *= 5
+    return r
+
+
+def test_gaussian_weight(lib):
+    """hss_gaussian_weight against trial division, and the weights the
+    reference quotes (evidence/benchmarks/heavy-benchmark.json: root 1116389
+    has weight 125, root 1447225 weight 1125; max 5625 at 9773725)."""
+    lib.hss_gaussian_weight.restype = ctypes.c_uint32
+    lib.hss_gaussian_weight.argtypes = [ctypes.c_uint64]
+    assert lib.hss_gaussian_weight(1116389) == 125
+    assert lib.hss_gaussian_weight(1447225) == 1125
+    assert lib.hss_gaussian_weight(9773725) == 5625
+    assert lib.hss_gaussian_weight(0) == 0 and lib.hss_gaussian_weight((1 << 26) + 1) == 0
+    import random
+    rng = random.Random(7)
+    xs = list(range(1, 3000)) + [rng.randrange(1, 1 << 26) for _ in range(300)] + [(1 << 26), 67108859]
+    for x in xs:
+        assert lib.hss_gaussian_weight(x) == _weight(x), x

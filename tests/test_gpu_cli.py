@@ -161,3 +161,87 @@ def test_search_coverage_skip(tmp_path):
     assert st["candidates"] == read_state(tmp_path / "s2")["candidates"]
     assert st["pairs"] == sum(5000 - l + 1 for l in range(3990, 4011))
     assert result_lines((tmp_path / "r").read_text()) == []
+
+
+def weight(x: int) -> int:
+    """Gaussian weight by trial division (experiments/heavy/README.md:9)."""
+    r, p = 1, 2
+    while p * p <= x:
+        e = 0
+        while x % p == 0:
+            x //= p
+            e += 1
+        if p % 4 == 1:
+            r *= 4 * e + 1
+        p += 1
+    if x > 1 and x % 4 == 1:
+        r *= 5
+    return r
+
+
+def check_done(state_path, first, resume, limit, min_weight):
+    """The checks experiments/heavy/verify_heavy.py:44-96 and
+    scripts/coverage_audit.py:71-99 make on a heavy checkpoint's .done file."""
+    lines = (state_path.parent / (state_path.name + ".done")).read_text().splitlines()
+    header, roots = {}, []
+    for ln in lines:
+        f = ln.split()
+        if len(f) == 2 and f[0] in {"limit", "min_weight"}:
+            header[f[0]] = int(f[1])
+        else:
+            assert len(f) == 1, ln
+            roots.append(int(f[0]))
+    assert header == {"limit": limit, "min_weight": min_weight}
+    assert all(a < b for a, b in zip(roots, roots[1:]))
+    assert roots == [l for l in range(first, resume) if weight(l) >= min_weight]
+    return roots
+
+
+def test_heavy_weight_mode(tmp_path):
+    """MIN_WEIGHT (experiments/heavy/gpu_search_v36.cu): batches of 4096,
+    min_weight in STATE with begin = FIRST_L, the .done file, and candidates
+    equal to searching exactly the heavy roots."""
+    import hss
+    limit, first, last, w = 70000, 1, 20000, 125
+    state, results = tmp_path / "heavy-v36-w125-1-20000-state.txt", tmp_path / "heavy-v36-w125-1-20000-results.txt"
+    r = run([GPU, limit, first, last, results, 0, state, w], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st["min_weight"] == w and st["begin"] == first and st["resume"] == last + 1
+    resumes = [int(ln.split()[2]) for ln in r.stdout.splitlines() if ln.startswith("PROGRESS")]
+    assert resumes == [min(first + 4096 * (i + 1), last + 1) for i in range(len(resumes))]
+    roots = check_done(state, first, last + 1, limit, w)
+    with hss.Context(limit) as ctx:
+        want = ctx.search(limit, roots)
+    assert st["candidates"] == want.total
+    assert result_lines(results.read_text()) == want.hits
+
+
+def test_heavy_weight_resume_and_mismatch(tmp_path):
+    import hss
+    limit, first, last, w = 70000, 1, 20000, 125
+    state = tmp_path / "h-state.txt"
+    # a checkpoint as a stopped worker leaves it after two batches
+    state.write_text(f"version 1\nlimit {limit}\nbegin {first}\nend {last}\nresume 8193\ncandidates 5\n"
+                     f"min_weight {w}\n")
+    r = run([GPU, limit, first, last, tmp_path / "r.txt", 0, state, w], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st["begin"] == first and st["resume"] == last + 1
+    roots = check_done(state, first, last + 1, limit, w)
+    with hss.Context(limit) as ctx:
+        assert st["candidates"] == ctx.search(limit, [l for l in roots if l >= 8193]).total
+    # another weight does not match the checkpoint
+    r = run([GPU, limit, first, last, tmp_path / "r.txt", 0, state, 45], tmp_path)
+    assert r.returncode == 0 and "checkpoint does not match requested range or weight" in r.stderr
+    assert read_state(state)["min_weight"] == w
+
+
+def test_min_weight_zero_keeps_512_batches(tmp_path):
+    state = tmp_path / "s.txt"
+    r = run([GPU, 5000, 1, 1200, tmp_path / "r.txt", 0, state, 0], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st["min_weight"] == 0 and st["resume"] == 1201
+    assert [int(ln.split()[2]) for ln in r.stdout.splitlines() if ln.startswith("PROGRESS")] == [513, 1025, 1201]
+    assert not (tmp_path / "s.txt.done").exists()
