@@ -672,19 +672,27 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
                     __builtin_amdgcn_wave_barrier();
                     // ---- phase 2: lane = item
                     const uint32_t icl = wave_incl_scan(cnt, lane);
+                    const uint32_t excl = icl - cnt;  // first item of this lane's partner
                     const uint32_t items = __builtin_amdgcn_readfirstlane(__shfl(icl, 63, 64));
+                    // Owners' items are contiguous and in lane order, so a step of 64
+                    // items spans few owners: a uniform cursor walks the owners whose
+                    // items start inside the step (scalar reads of their starts).
+                    uint32_t cur = 0;  // an owner whose items start at or before the step
                     for (uint32_t i0 = 0; i0 < items; i0 += 64) {
                         const uint32_t it = i0 + (uint32_t)lane;
                         const bool live = it < items;
-                        uint32_t o = 0;
-                        for (uint32_t s = 32; s > 0; s >>= 1) {
-                            uint32_t v = __shfl(icl, (int)(o + s - 1), 64);
-                            if (v <= it) o += s;
+                        cur = __builtin_amdgcn_readfirstlane(cur);  // uniform: keep it scalar
+                        uint32_t o = cur, so = __builtin_amdgcn_readlane(excl, cur);
+                        uint32_t nx = cur + 1;
+                        for (; nx < 64; ++nx) {
+                            const uint32_t snx = __builtin_amdgcn_readlane(excl, nx);
+                            if (snx >= i0 + 64) break;
+                            const bool past = it >= snx;
+                            o = past ? nx : o;
+                            so = past ? snx : so;
                         }
-                        o = o > 63 ? 63 : o;
-                        const uint32_t iclo = __shfl(icl, (int)o, 64);
-                        const uint32_t cnto = __shfl(cnt, (int)o, 64);
-                        const uint32_t q = live ? it - (iclo - cnto) : 0u;
+                        cur = nx - 1;
+                        const uint32_t q = live ? it - so : 0u;
                         const WinRec& Q = rec[o];
                         uint32_t pc = 0;
                         for (int p = 1; p < 8; ++p) pc += Q.cum[p] <= q ? 1u : 0u;
