@@ -62,6 +62,7 @@ std::vector<uint64_t> Sched::admit() {
         if (!pool_alloc(queue_.front().second, off)) break;  // FIFO: wait for memory
         Active a;
         a.npos = queue_.front().second;
+        a.stop = a.npos;
         a.pool_off = off;
         active_.emplace(queue_.front().first, std::move(a));
         out.push_back(queue_.front().first);
@@ -80,14 +81,27 @@ void Sched::set_costs(uint64_t id, std::vector<double> cost) {
 
 bool Sched::has_work() const {
     for (const auto& kv : active_)
-        if (kv.second.costed && kv.second.cursor < kv.second.npos) return true;
+        if (kv.second.costed && kv.second.cursor < kv.second.stop) return true;
     return false;
+}
+
+int Sched::cancel(uint64_t id) {
+    for (auto it = queue_.begin(); it != queue_.end(); ++it)
+        if (it->first == id) {
+            queue_.erase(it);
+            return 1;
+        }
+    auto it = active_.find(id);
+    if (it == active_.end()) return 0;
+    Active& a = it->second;
+    a.stop = a.cursor;
+    return a.done == a.stop ? 2 : 3;
 }
 
 std::vector<Segment> Sched::next_launch() {
     std::vector<std::pair<uint64_t, Active*>> live;
     for (auto& kv : active_)
-        if (kv.second.costed && kv.second.cursor < kv.second.npos) live.emplace_back(kv.first, &kv.second);
+        if (kv.second.costed && kv.second.cursor < kv.second.stop) live.emplace_back(kv.first, &kv.second);
     std::vector<Segment> segs;
     if (live.empty()) return segs;
     const double budget = cfg_.launch_work;
@@ -100,7 +114,7 @@ std::vector<Segment> Sched::next_launch() {
     for (size_t i = 0; i < live.size() && positions < cfg_.max_launch_positions; ++i) {
         Active& a = *live[i].second;
         double acc = 0;
-        while (a.cursor < a.npos && positions < cfg_.max_launch_positions) {
+        while (a.cursor < a.stop && positions < cfg_.max_launch_positions) {
             const double w = a.cost[a.cursor];
             if (a.cursor > start[i] && acc + w > share) break;
             acc += w;
@@ -112,7 +126,7 @@ std::vector<Segment> Sched::next_launch() {
     // pass 2: the rest of the budget in admission order
     for (size_t i = 0; i < live.size() && positions < cfg_.max_launch_positions; ++i) {
         Active& a = *live[i].second;
-        while (a.cursor < a.npos && positions < cfg_.max_launch_positions) {
+        while (a.cursor < a.stop && positions < cfg_.max_launch_positions) {
             const double w = a.cost[a.cursor];
             if (total + w > budget) break;
             total += w;
@@ -145,7 +159,7 @@ std::vector<uint64_t> Sched::finished(const std::vector<Segment>& segs) {
                 a.fin.erase(r);
             }
         }
-        if (a.done == a.npos) out.push_back(s.id);
+        if (a.done == a.stop) out.push_back(s.id);
     }
     return out;
 }

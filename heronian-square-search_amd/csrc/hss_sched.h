@@ -55,6 +55,11 @@ public:
     std::vector<uint64_t> finished(const std::vector<Segment>& segs);
     // Release a complete request's slot and pool range.
     void retire(uint64_t id);
+    // Stop scheduling a request: a queued one leaves the queue (returns 1); an
+    // admitted one keeps only the positions already handed to launches
+    // (returns 2 when they have all finished, so it completes now, else 3);
+    // 0 for an unknown id.
+    int cancel(uint64_t id);
 
     // first counter slot of an admitted request
     uint32_t pool_offset(uint64_t id) const;
@@ -69,6 +74,7 @@ public:
 private:
     struct Active {
         uint32_t npos = 0, pool_off = 0;
+        uint32_t stop = 0;    // positions to search (npos, or fewer after a cancel)
         uint32_t cursor = 0;  // positions handed to launches
         uint32_t done = 0;    // positions in finished launches
         bool costed = false;

@@ -45,14 +45,15 @@ namespace {
 struct ReqRec {
     uint64_t limit = 0, first = 0, last = 0;
     uint32_t npos = 0;
+    bool canceled = false;
     int32_t state = HSS_REQ_QUEUED;
     int32_t error = HSS_OK;
     uint32_t launches = 0;
     uint64_t resume = 0;
     double t_submit = -1, t_admit = -1, t_start = -1, t_done = -1;
-    std::vector<hss_root_result> per;
-    std::vector<HitOut> hits;
-    uint64_t cand = 0, leaves = 0, pairs = 0;
+    std::vector<hss_root_result> per;  // per position, filled as its launch finishes
+    std::vector<HitOut> hits;          // accepted hits found so far
+    uint64_t cand = 0, leaves = 0, pairs = 0;  // totals once finished
 };
 
 struct Launch {
@@ -71,7 +72,7 @@ struct hss_server {
     Counters cnt;
     uint64_t pool_positions = 0;
     LaunchBuffers bufs[2];
-    HostBuf readback;
+    HostBuf staging;  // pinned: counters of a finished launch
     Sched sched;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -109,29 +110,48 @@ void fail_all(hss_server* S, int code, const std::string& msg) {
     S->cv_done.notify_all();
 }
 
-// Copy a finished request's counters out of the pool and complete it.
-int complete_request(hss_server* S, uint64_t id) {
-    uint32_t off, npos;
+// Read the counters of a finished launch's positions out of the pool (on the
+// copy stream: the launch running meanwhile writes other positions).
+int read_back(hss_server* S, const std::vector<Segment>& segs) {
+    std::vector<std::pair<uint32_t, uint32_t>> src;  // (pool offset, count) per segment
+    size_t total = 0;
     {
         std::lock_guard<std::mutex> lk(S->mu);
-        off = S->sched.pool_offset(id);
-        npos = S->reqs[id].npos;
+        for (const Segment& sg : segs) {
+            src.emplace_back(S->sched.pool_offset(sg.id) + sg.begin, sg.end - sg.begin);
+            total += sg.end - sg.begin;
+        }
     }
-    HSS_HIP_TRY(S->readback.reserve((size_t)npos * 24));
-    unsigned long long* h = S->readback.as<unsigned long long>();
-    HSS_HIP_TRY(hipMemcpyAsync(h, S->cnt.cand + off, (size_t)npos * 8, hipMemcpyDeviceToHost, S->copy));
-    HSS_HIP_TRY(hipMemcpyAsync(h + npos, S->cnt.leaves + off, (size_t)npos * 8, hipMemcpyDeviceToHost, S->copy));
-    HSS_HIP_TRY(hipMemcpyAsync(h + 2 * (size_t)npos, S->cnt.pairs + off, (size_t)npos * 8, hipMemcpyDeviceToHost,
-                               S->copy));
+    HSS_HIP_TRY(S->staging.reserve(total * 24));
+    unsigned long long* h = S->staging.as<unsigned long long>();
+    size_t at = 0;
+    for (const auto& [off, n] : src) {
+        HSS_HIP_TRY(hipMemcpyAsync(h + at, S->cnt.cand + off, n * 8, hipMemcpyDeviceToHost, S->copy));
+        HSS_HIP_TRY(hipMemcpyAsync(h + total + at, S->cnt.leaves + off, n * 8, hipMemcpyDeviceToHost, S->copy));
+        HSS_HIP_TRY(hipMemcpyAsync(h + 2 * total + at, S->cnt.pairs + off, n * 8, hipMemcpyDeviceToHost, S->copy));
+        at += n;
+    }
     HSS_HIP_TRY(hipStreamSynchronize(S->copy));
     std::lock_guard<std::mutex> lk(S->mu);
+    at = 0;
+    for (const Segment& sg : segs) {
+        ReqRec& r = S->reqs[sg.id];
+        for (uint32_t i = sg.begin; i < sg.end; ++i, ++at)
+            r.per[i] = hss_root_result{h[at], h[total + at], h[2 * total + at]};
+    }
+    return HSS_OK;
+}
+
+// A request whose last scheduled positions have finished: totals, sorted
+// unique hits, release of its slot and pool range.  Caller holds S->mu.
+void finalize_locked(hss_server* S, uint64_t id) {
     ReqRec& r = S->reqs[id];
-    r.per.resize(npos);
-    for (uint32_t i = 0; i < npos; ++i) {
-        r.per[i] = hss_root_result{h[i], h[npos + i], h[2 * (size_t)npos + i]};
-        r.cand += h[i];
-        r.leaves += h[npos + i];
-        r.pairs += h[2 * (size_t)npos + i];
+    const uint32_t done = S->sched.done(id);
+    r.cand = r.leaves = r.pairs = 0;
+    for (uint32_t i = 0; i < r.npos; ++i) {
+        r.cand += r.per[i].candidates;
+        r.leaves += r.per[i].leaves;
+        r.pairs += r.per[i].pairs;
     }
     std::set<std::tuple<uint64_t, uint64_t, uint64_t>> seen;
     std::vector<HitOut> uniq;
@@ -141,16 +161,16 @@ int complete_request(hss_server* S, uint64_t id) {
         return std::tie(a.l, a.m, a.n) < std::tie(b.l, b.m, b.n);
     });
     r.hits.swap(uniq);
-    r.resume = r.last + 1;
-    r.state = HSS_REQ_DONE;
+    r.resume = r.first + S->sched.prefix(id);
+    r.state = r.canceled ? HSS_REQ_CANCELED : HSS_REQ_DONE;
     r.t_done = S->now();
     S->sched.retire(id);
     ++S->st.completed;
-    S->st.positions += npos;
+    S->st.positions += done;
     S->st.leaves += r.leaves;
     S->st.candidates += r.cand;
     S->cv_done.notify_all();
-    return HSS_OK;
+    S->cv_work.notify_all();
 }
 
 void worker_main(hss_server* S) {
@@ -178,6 +198,7 @@ void worker_main(hss_server* S) {
                 ReqRec& r = S->reqs[id];
                 r.state = HSS_REQ_ACTIVE;
                 r.t_admit = t;
+                r.per.assign(r.npos, hss_root_result{0, 0, 0});
                 todo.emplace_back(r.limit, r.first, r.npos);
             }
             S->st.max_active = std::max<uint32_t>(S->st.max_active, (uint32_t)S->sched.active());
@@ -260,26 +281,24 @@ void worker_main(hss_server* S) {
             return;
         }
         free_mask |= 1u << L.buf;
-        std::vector<uint64_t> done;
+        rc = read_back(S, L.segs);
+        if (rc != HSS_OK) {
+            fail_all(S, rc, hss_last_error());
+            (void)hipStreamSynchronize(c->stream);
+            return;
+        }
         {
             std::lock_guard<std::mutex> lk(S->mu);
             ++S->st.launches;
             S->st.gpu_ms += (double)R.setup_ms + R.cells_ms + R.exact_ms;
             for (const HitOut& h : R.hits)
                 if (h.pos < L.owner.size()) S->reqs[L.owner[h.pos]].hits.push_back(h);
-            done = S->sched.finished(L.segs);
+            const std::vector<uint64_t> done = S->sched.finished(L.segs);
             for (const Segment& sg : L.segs) {
                 ReqRec& r = S->reqs[sg.id];
                 r.resume = r.first + S->sched.prefix(sg.id);
             }
-        }
-        for (uint64_t id : done) {
-            rc = complete_request(S, id);
-            if (rc != HSS_OK) {
-                fail_all(S, rc, hss_last_error());
-                (void)hipStreamSynchronize(c->stream);
-                return;
-            }
+            for (uint64_t id : done) finalize_locked(S, id);
         }
     }
 }
@@ -307,7 +326,7 @@ extern "C" int hss_server_create(hss_ctx* ctx, const hss_server_config* cfg, hss
         if (S->copy) (void)hipStreamDestroy(S->copy);
         S->pool.release();
         for (LaunchBuffers& b : S->bufs) b.release();
-        S->readback.release();
+        S->staging.release();
         delete S;
         return fail(code, msg);
     };
@@ -364,10 +383,20 @@ static void fill_info(uint64_t id, const ReqRec& r, hss_request_info* info) {
     info->first = r.first;
     info->last = r.last;
     info->resume = r.resume;
-    info->candidates = r.cand;
-    info->leaves = r.leaves;
-    info->pairs = r.pairs;
-    info->n_hits = r.state == HSS_REQ_DONE ? (uint32_t)r.hits.size() : 0u;
+    const bool finished = r.state == HSS_REQ_DONE || r.state == HSS_REQ_CANCELED;
+    if (finished) {
+        info->candidates = r.cand;
+        info->leaves = r.leaves;
+        info->pairs = r.pairs;
+    } else {
+        const uint64_t prefix = r.per.empty() ? 0 : r.resume - r.first;
+        for (uint64_t i = 0; i < prefix; ++i) {
+            info->candidates += r.per[i].candidates;
+            info->leaves += r.per[i].leaves;
+            info->pairs += r.per[i].pairs;
+        }
+    }
+    info->n_hits = (uint32_t)r.hits.size();
     info->launches = r.launches;
     info->state = r.state;
     info->error = r.error;
@@ -382,7 +411,10 @@ extern "C" int hss_server_wait(hss_server* S, uint64_t id, int64_t timeout_ms, h
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(HSS_EINVAL, "unknown request id");
-    auto finished = [&] { return it->second.state == HSS_REQ_DONE || it->second.state == HSS_REQ_FAILED; };
+    auto finished = [&] {
+        const int32_t st = it->second.state;
+        return st == HSS_REQ_DONE || st == HSS_REQ_FAILED || st == HSS_REQ_CANCELED;
+    };
     if (timeout_ms < 0)
         S->cv_done.wait(lk, finished);
     else
@@ -401,13 +433,21 @@ extern "C" int hss_server_results(hss_server* S, uint64_t id, hss_root_result* p
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(HSS_EINVAL, "unknown request id");
     const ReqRec& r = it->second;
-    if (r.state != HSS_REQ_DONE) return fail(HSS_EAGAIN, "request not finished");
-    for (uint64_t i = 0; i < r.per.size() && i < cap; ++i) per_root[i] = r.per[i];
-    const uint32_t k = (uint32_t)r.hits.size();
+    const uint64_t npos = r.last - r.first + 1;
+    for (uint64_t i = 0; i < npos && i < cap; ++i) per_root[i] = i < r.per.size() ? r.per[i] : hss_root_result{0, 0, 0};
+    // hits found so far, unique and sorted (a finished request's list already is)
+    std::set<std::tuple<uint64_t, uint64_t, uint64_t>> seen;
+    std::vector<HitOut> uniq;
+    for (const HitOut& x : r.hits)
+        if (seen.insert({x.l, x.m, x.n}).second) uniq.push_back(x);
+    std::sort(uniq.begin(), uniq.end(), [](const HitOut& a, const HitOut& b) {
+        return std::tie(a.l, a.m, a.n) < std::tie(b.l, b.m, b.n);
+    });
+    const uint32_t k = (uint32_t)uniq.size();
     for (uint32_t i = 0; i < k && i < hit_cap; ++i)
-        hits[i] = hss_hit{r.hits[i].l, r.hits[i].m, r.hits[i].n, r.hits[i].area4_lo, r.hits[i].area4_hi};
+        hits[i] = hss_hit{uniq[i].l, uniq[i].m, uniq[i].n, uniq[i].area4_lo, uniq[i].area4_hi};
     if (n_hits) *n_hits = k;
-    if (cap < r.per.size() || k > hit_cap) return fail(HSS_EOVERFLOW, "result buffers too small");
+    if (cap < npos || k > hit_cap) return fail(HSS_EOVERFLOW, "result buffers too small");
     return HSS_OK;
 }
 
@@ -416,9 +456,32 @@ extern "C" int hss_server_release(hss_server* S, uint64_t id) {
     std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(HSS_EINVAL, "unknown request id");
-    if (it->second.state != HSS_REQ_DONE && it->second.state != HSS_REQ_FAILED)
+    const int32_t st = it->second.state;
+    if (st != HSS_REQ_DONE && st != HSS_REQ_FAILED && st != HSS_REQ_CANCELED)
         return fail(HSS_EAGAIN, "request not finished");
     S->reqs.erase(it);
+    return HSS_OK;
+}
+
+extern "C" int hss_server_cancel(hss_server* S, uint64_t id) {
+    if (!S) return fail(HSS_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(HSS_EINVAL, "unknown request id");
+    ReqRec& r = it->second;
+    if (r.state != HSS_REQ_QUEUED && r.state != HSS_REQ_ACTIVE) return HSS_OK;  // already finished
+    r.canceled = true;
+    const int how = S->sched.cancel(id);
+    if (how == 1) {  // never admitted
+        r.state = HSS_REQ_CANCELED;
+        r.resume = r.first;
+        r.t_done = S->now();
+        ++S->st.completed;
+        S->cv_done.notify_all();
+    } else if (how == 2) {  // every scheduled position has finished
+        finalize_locked(S, id);
+    }
+    // how == 3: finishes with the launches that hold its scheduled positions
     return HSS_OK;
 }
 
@@ -448,7 +511,7 @@ extern "C" void hss_server_destroy(hss_server* S) {
         (void)hipStreamDestroy(S->copy);
     }
     for (LaunchBuffers& b : S->bufs) b.release();
-    S->readback.release();
+    S->staging.release();
     S->pool.release();
     S->ctx->server = nullptr;
     delete S;

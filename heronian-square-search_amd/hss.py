@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libhss.so"
 
 HSS_OK, HSS_EINVAL, HSS_ENOMEM, HSS_EDEVICE, HSS_EOVERFLOW, HSS_EAGAIN = 0, 1, 2, 3, 4, 5
-REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED, REQ_CANCELED = 0, 1, 2, 3, 4
 
 
 class Hit(ctypes.Structure):
@@ -132,6 +132,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
     lib.hss_server_results.argtypes = [vp, u64, ctypes.POINTER(RootResult), u64, ctypes.POINTER(Hit), u32,
                                        ctypes.POINTER(u32)]
     lib.hss_server_release.argtypes = [vp, u64]
+    lib.hss_server_cancel.argtypes = [vp, u64]
     lib.hss_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     lib.hss_server_destroy.argtypes = [vp]
     lib.hss_server_destroy.restype = None
@@ -317,6 +318,8 @@ class Server:
 
     def result(self, rid: int, hit_cap: int = 4096) -> ServedResult:
         info = self.wait(rid)
+        if info is None or info["state"] == REQ_FAILED:
+            raise HssError(HSS_EDEVICE, "request failed")
         n = info["last"] - info["first"] + 1
         per = (RootResult * n)()
         hits = (Hit * max(hit_cap, 1))()
@@ -328,6 +331,22 @@ class Server:
 
     def release(self, rid: int):
         _check(_lib.hss_server_release(self._h, rid))
+
+    def cancel(self, rid: int):
+        """Stop a request; it ends REQ_CANCELED with resume at its finished prefix."""
+        _check(_lib.hss_server_cancel(self._h, rid))
+
+    def partial(self, rid: int, hit_cap: int = 4096) -> ServedResult:
+        """Results so far (finished positions and hits found so far), without waiting."""
+        info = self.info(rid)
+        n = info["last"] - info["first"] + 1
+        per = (RootResult * n)()
+        hits = (Hit * max(hit_cap, 1))()
+        nh = ctypes.c_uint32()
+        _check(_lib.hss_server_results(self._h, rid, per, n, hits, hit_cap, ctypes.byref(nh)))
+        return ServedResult(info, [per[i].candidates for i in range(n)], [per[i].leaves for i in range(n)],
+                            [per[i].pairs for i in range(n)],
+                            [(hits[i].l, hits[i].m, hits[i].n, hits[i].area) for i in range(nh.value)])
 
     def stats(self) -> dict:
         s = ServerStats()

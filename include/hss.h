@@ -212,15 +212,16 @@ typedef struct {
 #define HSS_REQ_ACTIVE 1 /* admitted: positions are being searched */
 #define HSS_REQ_DONE 2
 #define HSS_REQ_FAILED 3
+#define HSS_REQ_CANCELED 4 /* stopped by hss_server_cancel: [first, resume) finished */
 
 typedef struct {
     uint64_t id;
     uint64_t limit, first, last;
     uint64_t resume;     /* root sides [first, resume) are finished (a STATE file's resume) */
-    uint64_t candidates; /* totals, once done */
+    uint64_t candidates; /* of the finished prefix [first, resume); the totals once done */
     uint64_t leaves;
     uint64_t pairs;
-    uint32_t n_hits;     /* once done */
+    uint32_t n_hits;     /* hits found so far (raw count; unique ones via hss_server_results) */
     uint32_t launches;   /* launches the request's positions were part of */
     int32_t state;       /* HSS_REQ_* */
     int32_t error;       /* status of a failed request */
@@ -254,14 +255,22 @@ int hss_server_submit(hss_server* srv, uint64_t limit, uint64_t first, uint64_t 
  * error when it failed. */
 int hss_server_wait(hss_server* srv, uint64_t id, int64_t timeout_ms, hss_request_info* info);
 
-/* Results of a done request: per-position counters (cap entries, last-first+1
- * needed) and its hits (de-duplicated, sorted); HSS_EOVERFLOW when a buffer
- * is short (counts still reported). */
+/* Results of a request so far: per-position counters (cap entries, last-first+1
+ * needed; zero for positions not finished yet) and the hits found so far
+ * (de-duplicated, sorted); HSS_EOVERFLOW when a buffer is short (counts still
+ * reported).  A writer of the reference's files appends these hits before it
+ * writes a STATE resume taken from hss_server_wait. */
 int hss_server_results(hss_server* srv, uint64_t id, hss_root_result* per_root, uint64_t cap,
                        hss_hit* hits, uint32_t hit_cap, uint32_t* n_hits);
 
 /* Forget a finished request. */
 int hss_server_release(hss_server* srv, uint64_t id);
+
+/* Stop a request (the SIGTERM path of the reference's host loop,
+ * src/gpu_search_v42.cu:460-494): positions not yet handed to a launch are
+ * dropped; it ends HSS_REQ_CANCELED once the launches holding its other
+ * positions finish, with resume = the end of its finished prefix. */
+int hss_server_cancel(hss_server* srv, uint64_t id);
 
 int hss_server_get_stats(hss_server* srv, hss_server_stats* out);
 

@@ -388,6 +388,7 @@ uint32_t hc_sched_finished(void* h, const uint64_t* ids, const uint32_t* begin, 
 }
 
 void hc_sched_retire(void* h, uint64_t id) { static_cast<Sched*>(h)->retire(id); }
+int hc_sched_cancel(void* h, uint64_t id) { return static_cast<Sched*>(h)->cancel(id); }
 uint32_t hc_sched_pool_offset(void* h, uint64_t id) { return static_cast<Sched*>(h)->pool_offset(id); }
 uint32_t hc_sched_prefix(void* h, uint64_t id) { return static_cast<Sched*>(h)->prefix(id); }
 uint64_t hc_sched_pool_free(void* h) { return static_cast<Sched*>(h)->pool_free(); }

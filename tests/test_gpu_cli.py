@@ -22,6 +22,7 @@ from conftest import PKG, result_lines
 pytestmark = pytest.mark.gpu
 
 GPU = PKG / "gpu_search"
+SERVE = PKG / "gpu_serve"
 CPU = PKG / "search"
 CASES = {
     "full5000": dict(limit=5000, first=1, last=5000, pairs=12502500, candidates=151842556,
@@ -245,3 +246,70 @@ def test_min_weight_zero_keeps_512_batches(tmp_path):
     assert st["min_weight"] == 0 and st["resume"] == 1201
     assert [int(ln.split()[2]) for ln in r.stdout.splitlines() if ln.startswith("PROGRESS")] == [513, 1025, 1201]
     assert not (tmp_path / "s.txt.done").exists()
+
+
+def serve(tmp_path, lines, **kw):
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(tmp_path / "none.json"))
+    return subprocess.run([str(SERVE), "100000", "0"], input="".join(l + "\n" for l in lines), cwd=tmp_path,
+                          env=env, capture_output=True, text=True, timeout=600, **kw)
+
+
+def test_gpu_serve_files(tmp_path, golden):
+    """The served loop writes per request what one gpu_search_v42 process
+    writes for its interval: STATE, results lines and a log whose PROGRESS
+    chain steps by 512 from FIRST to LAST + 1 (scripts/coverage_audit.py:48-68)."""
+    reqs = {"gpu-0a": (5000, 1, 5000), "gpu-1a": (70000, 11789, 11789), "gpu-2a": (100000, 99901, 100000)}
+    r = serve(tmp_path, [f"{a} {b} {c} {p}" for p, (a, b, c) in reqs.items()])
+    assert r.returncode == 0, r.stderr
+    done = {ln.split()[1]: ln for ln in r.stdout.splitlines() if ln.startswith("DONE")}
+    assert set(done) == set(reqs)
+    want = {"gpu-0a": golden["full5000"], "gpu-1a": golden["known2"], "gpu-2a": golden["i100000_99901_100000"]}
+    for p, (limit, first, last) in reqs.items():
+        st = read_state(tmp_path / f"{p}-state.txt")
+        assert st == {"version": 1, "limit": limit, "begin": first, "end": last, "resume": last + 1,
+                      "candidates": want[p]["candidates"]}
+        assert result_lines((tmp_path / f"{p}-results.txt").read_text()) == result_lines(want[p]["results"])
+        assert chain_end((tmp_path / f"{p}.log").read_text(), first, last) == last + 1
+        assert f"resume {last + 1} candidates {want[p]['candidates']}" in done[p]
+
+
+def test_gpu_serve_checkpoints(tmp_path):
+    """Resume from a STATE on the +512 chain, mismatching STATE skipped,
+    SIGTERM stops with a STATE on the chain, and the rerun completes it."""
+    # a resumed request: the chain continues from the STATE's resume
+    (tmp_path / "r-state.txt").write_text("version 1\nlimit 5000\nbegin 1\nend 3000\nresume 1025\ncandidates 9\n")
+    (tmp_path / "m-state.txt").write_text("version 1\nlimit 5000\nbegin 1\nend 10\nresume 3\ncandidates 0\n")
+    r = serve(tmp_path, ["5000 1 3000 r", "5000 1 20 m"])
+    assert r.returncode == 0, r.stderr
+    assert "m: checkpoint does not match requested range" in r.stderr
+    st = read_state(tmp_path / "r-state.txt")
+    assert st["begin"] == 1025 and st["resume"] == 3001
+    assert chain_end((tmp_path / "r.log").read_text(), 1025, 3000) == 3001
+    import hss
+    with hss.Context(5000) as ctx:
+        assert st["candidates"] == ctx.search(5000, range(1025, 3001)).total
+    # SIGTERM while a long request runs
+    limit, first, last = 1_000_000, 1, 30_000
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(tmp_path / "none.json"))
+    p = subprocess.Popen([str(SERVE), str(limit), "0"], stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, cwd=tmp_path, env=env, text=True)
+    p.stdin.write(f"{limit} {first} {last} big\n")
+    p.stdin.flush()
+    log = tmp_path / "big.log"
+    deadline = time.time() + 120
+    while time.time() < deadline and not (log.exists() and "PROGRESS" in log.read_text()):
+        time.sleep(0.02)
+    p.send_signal(signal.SIGTERM)
+    out, err = p.communicate(timeout=300)
+    assert p.returncode == 0, err
+    stopped = read_state(tmp_path / "big-state.txt")
+    assert (stopped["resume"] - first) % 512 == 0 or stopped["resume"] == last + 1
+    r = subprocess.run([str(SERVE), str(limit), "0"], input=f"{limit} {first} {last} big\n", cwd=tmp_path,
+                       env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    final = read_state(tmp_path / "big-state.txt")
+    assert final["resume"] == last + 1
+    assert chain_end(log.read_text(), first, last) == last + 1
+    with hss.Context(limit) as ctx:
+        total = ctx.search_interval(limit, first, last)[0]
+    assert stopped["candidates"] + final["candidates"] == total

@@ -170,3 +170,35 @@ def test_seeded_stream_from_threads(hss_mod, ctx):
     for i, req in enumerate(reqs):
         ref = ctx.search(req[0], range(req[1], req[2] + 1))
         served_equals_alone(served[i], ref)
+
+
+def test_cancel_and_partial(hss_mod, ctx):
+    """A request stopped while it runs ends REQ_CANCELED; its finished prefix
+    [first, resume) has the alone counts; partial results while running show
+    finished positions only."""
+    big = (1_000_000, 1, 20_000)
+    with hss_mod.Server(ctx, launch_work=2e9) as srv:
+        rid = srv.submit(*big)
+        small = srv.submit(70000, 11789, 11789)
+        seen_partial = None
+        for _ in range(400):
+            info = srv.info(rid)
+            if info["resume"] > big[1] + 100:
+                seen_partial = srv.partial(rid)
+                break
+            time.sleep(0.005)
+        srv.cancel(rid)
+        info = srv.wait(rid)
+        assert info["state"] == hss_mod.REQ_CANCELED
+        assert big[1] <= info["resume"] <= big[2] + 1
+        res = srv.result(small)
+        assert res.total == 1_194_756
+        part = srv.partial(rid)
+    n_done = info["resume"] - big[1]
+    if n_done:
+        alone = ctx.search(big[0], range(big[1], info["resume"]))
+        assert part.candidates[:n_done] == alone.candidates
+        assert info["candidates"] == alone.total
+    if seen_partial is not None:
+        k = seen_partial.info["resume"] - big[1]
+        assert seen_partial.candidates[:k] == part.candidates[:k]

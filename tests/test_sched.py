@@ -39,6 +39,8 @@ def hc():
                                       ctypes.POINTER(u64), u32]
     lib.hc_sched_finished.restype = u32
     lib.hc_sched_retire.argtypes = [vp, u64]
+    lib.hc_sched_cancel.argtypes = [vp, u64]
+    lib.hc_sched_cancel.restype = ctypes.c_int
     for name in ("hc_sched_pool_offset", "hc_sched_prefix"):
         getattr(lib, name).argtypes = [vp, u64]
         getattr(lib, name).restype = u32
@@ -255,4 +257,29 @@ def test_equal_share_keeps_short_requests_moving(hc):
     first = [x for x in segs if x[0] == 1][0]
     assert first[2] - first[1] == 95  # its share (50) plus the rest of the budget
     assert s.finished(segs) == [2]
+    s.close()
+
+
+def test_cancel(hc):
+    """Cancel: a queued request leaves the queue; an admitted one keeps only
+    the positions already handed to launches and completes when they finish."""
+    s = Sched(hc, 4, 1000, 10.0)
+    for rid, n in [(1, 100), (2, 50), (3, 10)]:
+        s.enqueue(rid, n)
+    assert s.admit() == [1, 2, 3]
+    for rid, n in [(1, 100), (2, 50), (3, 10)]:
+        s.set_costs(rid, [1.0] * n)
+    s.enqueue(4, 5)
+    assert s.cancel(4) == 1 and s.queued() == 0   # never admitted
+    assert s.cancel(3) == 2                       # nothing handed out yet: complete now
+    s.retire(3)
+    a = s.next()                                  # 1 and 2 get work
+    assert {x[0] for x in a} == {1, 2}
+    assert s.cancel(1) == 3                       # positions of `a` are in flight
+    b = s.next()
+    assert {x[0] for x in b} == {2}               # 1 gets no more work
+    done = s.finished(a)
+    assert 1 in done
+    assert s.prefix(1) == [x for x in a if x[0] == 1][0][2]
+    assert s.cancel(99) == 0
     s.close()
