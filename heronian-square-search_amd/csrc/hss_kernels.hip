@@ -551,17 +551,19 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
 //     with leaf_fast from its owner's record and a vector load of u.
 // Leaves outside the ranges count 0 exactly and are skipped.
 
+// 112 bytes: four blocks of four waves fit a CU's LDS with the root queues.
+// The narrowed range bounds are derived from the widened ones (n = w -+ 2 tau).
 struct WinRec {
     double c, d;  // partner v
-    double t1_lo_w, t1_lo_n, t1_hi_w, t1_hi_n, t2_w, t2_n, tau, lm;
+    double t1_lo_w, t1_hi_w, t2_w, tau, lm;
     uint32_t inv_h, thr_h;
     uint32_t cres;   // (c mod 180) | (d mod 180) << 16
     uint32_t vidx;
     uint32_t kj;
-    uint32_t pad;
     uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
     uint16_t cum[8];  // items before each range
 };
+static_assert(sizeof(WinRec) == 112, "LDS budget of cell_win_kernel");
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
     __shared__ double s_qz[WAVES][QCAP];
@@ -648,11 +650,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
                     R.c = c;
                     R.d = d;
                     R.t1_lo_w = ctx.t1_lo_w;
-                    R.t1_lo_n = ctx.t1_lo_n;
                     R.t1_hi_w = ctx.t1_hi_w;
-                    R.t1_hi_n = ctx.t1_hi_n;
                     R.t2_w = ctx.t2_w;
-                    R.t2_n = ctx.t2_n;
                     R.tau = ctx.tau;
                     R.lm = ctx.lm;
                     R.inv_h = ctx.inv_h;
@@ -660,7 +659,6 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
                     R.cres = vr;
                     R.vidx = vidx;
                     R.kj = kj;
-                    R.pad = 0;
                     uint32_t acc = 0;
                     for (int p = 0; p < 8; ++p) {
                         const bool live = p < 4 ? n0 != 0 : n1 != 0;
@@ -705,12 +703,12 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
                         const uint32_t qc180 = qr & 0xFFFFu, qd180 = qr >> 16;
                         LaneCtx qx;
                         qx.t1_lo_w = Q.t1_lo_w;
-                        qx.t1_lo_n = Q.t1_lo_n;
                         qx.t1_hi_w = Q.t1_hi_w;
-                        qx.t1_hi_n = Q.t1_hi_n;
                         qx.t2_w = Q.t2_w;
-                        qx.t2_n = Q.t2_n;
                         qx.tau = Q.tau;
+                        qx.t1_lo_n = __builtin_fma(2.0, Q.tau, Q.t1_lo_w);
+                        qx.t1_hi_n = __builtin_fma(-2.0, Q.tau, Q.t1_hi_w);
+                        qx.t2_n = __builtin_fma(-2.0, Q.tau, Q.t2_w);
                         qx.lm = Q.lm;
                         qx.inv_h = Q.inv_h;
                         qx.thr_h = Q.thr_h;

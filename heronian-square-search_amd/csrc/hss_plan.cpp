@@ -3,7 +3,9 @@
 #include "hss_plan.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 
 namespace hss {
 
@@ -66,6 +68,28 @@ bool factor_root(uint64_t l, const std::vector<uint32_t>& primes, RootFactors& f
 
 // partners' half-set entries with index in [a, b]
 inline uint64_t entries(const uint32_t* off, uint32_t a, uint32_t b) { return (uint64_t)off[b + 1] - off[a]; }
+
+// Run f(begin, end, chunk) over `chunks` contiguous pieces of [0, n) on up to
+// 16 host threads (pieces taken dynamically; results are merged by the caller
+// in chunk order, so the plan does not depend on the thread count).
+template <typename F>
+void parallel_chunks(uint32_t n, uint32_t chunks, F&& f) {
+    const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint32_t nt = std::min(hw, chunks);
+    std::atomic<uint32_t> next{0};
+    auto run = [&] {
+        for (uint32_t c; (c = next.fetch_add(1)) < chunks;)
+            f((uint32_t)((uint64_t)n * c / chunks), (uint32_t)((uint64_t)n * (c + 1) / chunks), c);
+    };
+    if (nt <= 1) {
+        run();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(run);
+    run();
+    for (std::thread& t : pool) t.join();
+}
 
 // Visit every signature of root l: exponent vectors s_j >= 0 over l's split
 // primes with q = prod p_j^s_j <= limit.  visit(q, sexp) is called once each.
@@ -145,93 +169,125 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                 uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& P, std::string& err) {
     P = Plan();
     P.pos.resize(n_roots);
-    for (uint32_t s = 0; s < n_roots; ++s) {
-        const uint64_t l = reqs[s].l, limit = reqs[s].limit;
-        if (limit < 1 || limit > n_max) {
-            err = "limit outside [1, n_max]";
-            return false;
-        }
-        if (l < 1 || l > limit) {
-            err = "root outside [1, limit]";
-            return false;
-        }
-        RootFactors f;
-        if (!factor_root(l, small_primes, f)) {
-            err = "root has too many distinct prime factors";
-            return false;
-        }
-        PosDesc& D = P.pos[s];
-        D = PosDesc();
-        D.l = l;
-        D.limit = limit;
-        const double nd = (double)(limit * limit);
-        D.n4_d = nd * nd;
-        const double l2 = (double)(l * l);
-        D.l2_d = l2;
-        D.l4_d = l2 * l2;
-        D.l3 = (uint32_t)(l % 3);
-        D.l5 = (uint32_t)(l % 5);
-        D.l_even = (l & 1) ? 0u : 1u;
-        D.nsplit = f.ns;
-        D.ninert = f.ni;
-        D.out = reqs[s].out;
-        for (uint32_t j = 0; j < f.ns; ++j) {
-            D.split_inv[j] = inv_mod_2_32(f.sp[j]);
-            D.split_thr[j] = 0xFFFFFFFFu / f.sp[j];
-        }
-        for (uint32_t j = 0; j < f.ni; ++j) {
-            D.inert_inv[j] = inv_mod_2_32(f.ip[j]);
-            D.inert_thr[j] = 0xFFFFFFFFu / f.ip[j];
-        }
-        for_each_signature(f, limit, [&](uint64_t q, const uint32_t* sexp) {
-            uint64_t kmin = (l + q - 1) / q, kmax = limit / q;
-            if (kmin > kmax) return;
-            LeftBuild B{};
-            B.fixed = f.fixed;
-            B.k = f.ns;
-            for (uint32_t j = 0; j < f.ns; ++j) {
-                B.p[j] = f.sp[j];
-                B.E[j] = f.se[j] + sexp[j];
+    // positions: factorisation, signatures and left-set recipes, in chunks
+    struct PosChunk {
+        std::vector<LeftDesc> lefts;
+        std::vector<LeftBuild> builds;
+        std::vector<uint32_t> kmin, kmax;
+        uint64_t entries = 0;
+        double cells = 0;
+        std::string err;
+    };
+    const uint32_t pchunks = std::max(1u, std::min(64u, n_roots / 16));
+    std::vector<PosChunk> pc(pchunks);
+    parallel_chunks(n_roots, pchunks, [&](uint32_t b, uint32_t e, uint32_t c) {
+        PosChunk& C = pc[c];
+        for (uint32_t s = b; s < e && C.err.empty(); ++s) {
+            const uint64_t l = reqs[s].l, limit = reqs[s].limit;
+            if (limit < 1 || limit > n_max) {
+                C.err = "limit outside [1, n_max]";
+                break;
             }
-            LeftDesc L{};
-            L.pos = s;
-            L.off = (uint32_t)P.left_entries;
-            L.H = (uint32_t)left_size(f, sexp);
-            L.q = (uint32_t)q;
-            L.trivial = f.ns == 0 ? 1u : 0u;
-            L.q3 = (uint32_t)(q % 3);
-            L.q5 = (uint32_t)(q % 5);
-            P.left_entries += L.H;
-            P.lefts.push_back(L);
-            P.builds.push_back(B);
-            P.kmin.push_back((uint32_t)kmin);
-            P.kmax.push_back((uint32_t)kmax);
-            P.est_cells += (double)L.H * (double)entries(off, (uint32_t)kmin, (uint32_t)kmax);
-        });
+            if (l < 1 || l > limit) {
+                C.err = "root outside [1, limit]";
+                break;
+            }
+            RootFactors f;
+            if (!factor_root(l, small_primes, f)) {
+                C.err = "root has too many distinct prime factors";
+                break;
+            }
+            PosDesc& D = P.pos[s];
+            D = PosDesc();
+            D.l = l;
+            D.limit = limit;
+            const double nd = (double)(limit * limit);
+            D.n4_d = nd * nd;
+            const double l2 = (double)(l * l);
+            D.l2_d = l2;
+            D.l4_d = l2 * l2;
+            D.l3 = (uint32_t)(l % 3);
+            D.l5 = (uint32_t)(l % 5);
+            D.l_even = (l & 1) ? 0u : 1u;
+            D.nsplit = f.ns;
+            D.ninert = f.ni;
+            D.out = reqs[s].out;
+            for (uint32_t j = 0; j < f.ns; ++j) {
+                D.split_inv[j] = inv_mod_2_32(f.sp[j]);
+                D.split_thr[j] = 0xFFFFFFFFu / f.sp[j];
+            }
+            for (uint32_t j = 0; j < f.ni; ++j) {
+                D.inert_inv[j] = inv_mod_2_32(f.ip[j]);
+                D.inert_thr[j] = 0xFFFFFFFFu / f.ip[j];
+            }
+            for_each_signature(f, limit, [&](uint64_t q, const uint32_t* sexp) {
+                uint64_t kmin = (l + q - 1) / q, kmax = limit / q;
+                if (kmin > kmax) return;
+                LeftBuild B{};
+                B.fixed = f.fixed;
+                B.k = f.ns;
+                for (uint32_t j = 0; j < f.ns; ++j) {
+                    B.p[j] = f.sp[j];
+                    B.E[j] = f.se[j] + sexp[j];
+                }
+                LeftDesc L{};
+                L.pos = s;
+                L.off = (uint32_t)C.entries;  // relative to the chunk until merged
+                L.H = (uint32_t)left_size(f, sexp);
+                L.q = (uint32_t)q;
+                L.trivial = f.ns == 0 ? 1u : 0u;
+                L.q3 = (uint32_t)(q % 3);
+                L.q5 = (uint32_t)(q % 5);
+                C.entries += L.H;
+                C.lefts.push_back(L);
+                C.builds.push_back(B);
+                C.kmin.push_back((uint32_t)kmin);
+                C.kmax.push_back((uint32_t)kmax);
+                C.cells += (double)L.H * (double)entries(off, (uint32_t)kmin, (uint32_t)kmax);
+            });
+        }
+    });
+    for (PosChunk& C : pc) {
+        if (!C.err.empty()) {
+            err = C.err;
+            return false;
+        }
+        if (P.left_entries + C.entries > 0xFFFFFFFFull) {
+            err = "batch too large: split it";
+            return false;
+        }
+        for (LeftDesc& L : C.lefts) L.off += (uint32_t)P.left_entries;
+        P.left_entries += C.entries;
+        P.est_cells += C.cells;
+        P.lefts.insert(P.lefts.end(), C.lefts.begin(), C.lefts.end());
+        P.builds.insert(P.builds.end(), C.builds.begin(), C.builds.end());
+        P.kmin.insert(P.kmin.end(), C.kmin.begin(), C.kmin.end());
+        P.kmax.insert(P.kmax.end(), C.kmax.begin(), C.kmax.end());
     }
-    if (P.left_entries > 0xFFFFFFFFull) {
-        err = "batch too large: split it";
-        return false;
-    }
+    pc.clear();
     // Window mode per left set: eligible by size, then per tile by the
     // window fraction of its partner range.  Work is estimated in plain-leaf
     // units: a plain cell costs 1, a window-mode cell f * win_item_cost plus
     // win_partner_cost / H per partner entry.
     const uint32_t nl = (uint32_t)P.lefts.size();
+    const uint32_t lchunks = std::max(1u, std::min(256u, nl / 64));
     std::vector<uint8_t> eligible(nl, 0);
     std::vector<double> gain(nl, 1.0);  // cost per cell of the whole left set
     std::vector<double> fmax(nl, 0.0);
-    for (uint32_t li = 0; li < nl; ++li) {
-        const LeftDesc& L = P.lefts[li];
-        const uint32_t H = L.H;
-        const int wc = weight_class(H);
-        if (wc < 0 || !(tune.win_max_frac[wc] > 0.0)) continue;
-        eligible[li] = 1;
-        fmax[li] = tune.win_max_frac[wc];
-        const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, P.kmin[li], P.kmax[li], 9);
-        if (f <= fmax[li]) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
-        if (gain[li] > 1.0) gain[li] = 1.0;
-    }
+    parallel_chunks(nl, lchunks, [&](uint32_t b, uint32_t e, uint32_t) {
+        for (uint32_t li = b; li < e; ++li) {
+            const LeftDesc& L = P.lefts[li];
+            const uint32_t H = L.H;
+            const int wc = weight_class(H);
+            if (wc < 0 || !(tune.win_max_frac[wc] > 0.0)) continue;
+            eligible[li] = 1;
+            fmax[li] = tune.win_max_frac[wc];
+            const double f =
+                tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, P.kmin[li], P.kmax[li], 9);
+            if (f <= fmax[li]) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
+            if (gain[li] > 1.0) gain[li] = 1.0;
+        }
+    });
     double total = 0;
     for (uint32_t li = 0; li < nl; ++li)
         total += (double)P.lefts[li].H * (double)entries(off, P.kmin[li], P.kmax[li]) * gain[li];
@@ -244,44 +300,63 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
         uint32_t i;
         bool win;
     };
+    struct TileChunk {
+        std::vector<TileDesc> tiles;
+        std::vector<Cost> costs;
+        double win_cells = 0, work = 0;
+    };
+    std::vector<TileChunk> tc(lchunks);
+    std::vector<uint8_t> sort_me(nl, 0);
+    parallel_chunks(nl, lchunks, [&](uint32_t b, uint32_t e, uint32_t c) {
+        TileChunk& C = tc[c];
+        for (uint32_t li = b; li < e; ++li) {
+            const LeftDesc& L = P.lefts[li];
+            uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
+            uint64_t budget = (uint64_t)std::max(target / ((double)L.H * gain[li]), 256.0);
+            while (k0 <= kmax) {
+                // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
+                uint32_t lo = k0, hi = kmax;
+                while (lo < hi) {
+                    uint32_t mid = lo + (hi - lo + 1) / 2;
+                    if (entries(off, k0, mid) <= budget)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
+                const double cells = (double)L.H * (double)entries(off, k0, kend);
+                double cost = cells;
+                bool win = false;
+                if (eligible[li]) {
+                    const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);
+                    if (f <= fmax[li]) {
+                        win = true;
+                        cost = cells * f * tune.win_item_cost +
+                               (double)entries(off, k0, kend) * tune.win_partner_cost;
+                        sort_me[li] = 1;
+                        C.win_cells += cells;
+                    }
+                }
+                C.work += cost;
+                C.costs.push_back({cost, (uint32_t)C.tiles.size(), win});
+                C.tiles.push_back(TileDesc{li, k0, kend, 0});
+                if (kend == kmax) break;
+                k0 = kend + 1;
+            }
+        }
+    });
     std::vector<Cost> order;
     std::vector<TileDesc> all;
-    std::vector<uint8_t> sort_me(nl, 0);
-    for (uint32_t li = 0; li < nl; ++li) {
-        const LeftDesc& L = P.lefts[li];
-        uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
-        uint64_t budget = (uint64_t)std::max(target / ((double)L.H * gain[li]), 256.0);
-        while (k0 <= kmax) {
-            // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
-            uint32_t lo = k0, hi = kmax;
-            while (lo < hi) {
-                uint32_t mid = lo + (hi - lo + 1) / 2;
-                if (entries(off, k0, mid) <= budget)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
-            const double cells = (double)L.H * (double)entries(off, k0, kend);
-            double cost = cells;
-            bool win = false;
-            if (eligible[li]) {
-                const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);
-                if (f <= fmax[li]) {
-                    win = true;
-                    cost = cells * f * tune.win_item_cost +
-                           (double)entries(off, k0, kend) * tune.win_partner_cost;
-                    sort_me[li] = 1;
-                    P.win_cells += cells;
-                }
-            }
-            P.est_work += cost;
-            order.push_back({cost, (uint32_t)all.size(), win});
-            all.push_back(TileDesc{li, k0, kend, 0});
-            if (kend == kmax) break;
-            k0 = kend + 1;
+    for (TileChunk& C : tc) {
+        for (Cost& x : C.costs) {
+            x.i += (uint32_t)all.size();
+            order.push_back(x);
         }
+        all.insert(all.end(), C.tiles.begin(), C.tiles.end());
+        P.win_cells += C.win_cells;
+        P.est_work += C.work;
     }
+    tc.clear();
     for (uint32_t li = 0; li < nl; ++li) {
         P.lefts[li].bt = NO_TABLE;
         if (sort_me[li]) {
@@ -289,9 +364,23 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
             P.sorted.push_back(li);
         }
     }
-    // heaviest tiles first, per mode
-    std::sort(order.begin(), order.end(), [](const Cost& a, const Cost& b) { return a.c > b.c; });
-    for (const Cost& c : order) (c.win ? P.wtiles : P.tiles).push_back(all[c.i]);
+    // heaviest tiles first, per mode: a counting sort on the cost quantised to
+    // 1/8 octaves (load balance needs the order only roughly; linear time)
+    constexpr int KEYS = 8 * 64;
+    auto key = [](double c) {
+        const int k = c > 1.0 ? (int)(8.0 * std::log2(c)) : 0;
+        return KEYS - 1 - std::min(k, KEYS - 1);  // descending cost
+    };
+    std::vector<uint32_t> start(KEYS + 1, 0);
+    for (const Cost& c : order) ++start[key(c.c) + 1];
+    for (int k = 0; k < KEYS; ++k) start[k + 1] += start[k];
+    std::vector<uint32_t> sorted(order.size());
+    for (uint32_t i = 0; i < order.size(); ++i) sorted[start[key(order[i].c)]++] = i;
+    size_t nwin = 0;
+    for (const Cost& c : order) nwin += c.win ? 1 : 0;
+    P.wtiles.reserve(nwin);
+    P.tiles.reserve(order.size() - nwin);
+    for (uint32_t i : sorted) (order[i].win ? P.wtiles : P.tiles).push_back(all[order[i].i]);
     return true;
 }
 
