@@ -333,12 +333,53 @@ static constexpr int WG = 256;
 static constexpr int WAVES = WG / 64;
 static constexpr int QCAP = 128;
 
-__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
-    for (int s = 1; s < 64; s <<= 1) {
-        uint32_t t = __shfl_up(v, (unsigned)s, 64);
-        if (lane >= s) v += t;
-    }
+// Inclusive prefix sum / maximum over the wavefront with DPP: shifts by 1, 2,
+// 4, 8 within each row of 16 lanes, then the last lane of rows 0 and 2 into
+// rows 1 and 3, and the last lane of row 1 into rows 2 and 3 (lanes a DPP move
+// does not reach keep the identity 0).
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t dpp_shr(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    v += dpp_shr<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_shr<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_shr<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_shr<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_shr<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+    v += dpp_shr<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
     return v;
+}
+
+__device__ inline uint32_t wave_incl_max(uint32_t v) {
+    v = max(v, dpp_shr<0x111, 0xf>(v));
+    v = max(v, dpp_shr<0x112, 0xf>(v));
+    v = max(v, dpp_shr<0x114, 0xf>(v));
+    v = max(v, dpp_shr<0x118, 0xf>(v));
+    v = max(v, dpp_shr<0x142, 0xa>(v));
+    v = max(v, dpp_shr<0x143, 0xc>(v));
+    return v;
+}
+
+// Owners of flattened items.  Lane j owns items [start_j, start_j + len_j),
+// starts ascending with j.  For the step of items [i0, i0 + 64), the owner of
+// item i0 + lane is the last lane whose nonempty run starts at or before it:
+// starts inside the step are marked in a per-wave LDS array (65 words; slot 64
+// takes the lanes without a mark) and a prefix maximum spreads them; carry is
+// the owner of item i0 - 1 (uniform, updated to the owner of item i0 + 63).
+__device__ inline uint32_t step_owner(uint32_t* mark, uint32_t start, uint32_t len, uint32_t i0, uint32_t& carry) {
+    const uint32_t lane = __lane_id();
+    mark[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t d = start - i0;
+    mark[len != 0 && d < 64 ? d : 64u] = lane + 1;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t m = wave_incl_max(mark[lane]);
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t o = m ? m - 1 : carry;
+    carry = __builtin_amdgcn_readlane(o, 63);
+    return o;
 }
 
 __device__ inline uint64_t wave_sum64(uint64_t v) {
@@ -354,7 +395,7 @@ __device__ inline bool divides(uint32_t k, uint32_t inv, uint32_t thr) { return 
 
 // Append the lanes with pred set to the global deferred buffer.
 __device__ inline void push_global(const CellParams& P, bool pred, const DefItem& it) {
-    uint64_t mask = __ballot(pred);
+    uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
     // Every lane executes the atomic (lane 0 adds the count, the others 0) and the
     // base is read with readfirstlane: no divergent branch may precede a
@@ -366,175 +407,265 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
     if (pred && slot < P.def_cap) P.def[slot] = it;
 }
 
-// Per-wave LDS queue of counted terms awaiting the root-proximity test.
+// Per-wave LDS queue of leaves for the deferred buffer: leaves with counted
+// terms awaiting the root-proximity test, and leaves the FP64 band left
+// undecided.  Entry: X, Y of z/2 and the packed id: (u | partner entry << 17,
+// k | conj << 27 | counted terms << 28), the terms as bits 0-3 for lm - 2|X|,
+// lm + 2|X|, lm - 2|Y|, lm + 2|Y|; an entry without counted terms is undecided.
 struct RootQueue {
-    double* z;
-    uint4* id;  // (ui, vidx, k | conj << 30, unused)
+    double2* xy;
+    uint2* id;
     uint32_t n;
 };
 
-__device__ inline void root_drain(const CellParams& P, RootQueue& q, uint32_t left, uint32_t count) {
+// What a drain needs besides the queue: the tile's left set and position.
+struct TileCtx {
+    uint32_t left;
+    uint32_t q;      // m = q * k
+    double l4_d;     // FP64 l^4
+};
+
+__device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj << ID_U_BITS; }
+
+// The first `count` queued leaves, one per lane: an undecided leaf goes to the
+// deferred buffer for the full exact count; a leaf with counted terms goes
+// there when one of their zz estimates passes the root-proximity test.  The
+// wave tests one term per lane at a time, as many rounds as the most terms
+// any lane has.
+__device__ inline void root_drain(const CellParams& P, const RootQueue& q, const TileCtx& tc, uint32_t count) {
     const uint32_t lane = __lane_id();
     const bool in = lane < count;
     const uint32_t at = in ? lane : 0u;  // branch-free: every lane reads a valid slot
-    const uint4 id = q.id[at];
-    const bool pass = in && root_near_integer(q.z[at]);
-    push_global(P, pass, DefItem{left, id.x, id.y, id.z});
+    const double2 xy = q.xy[at];
+    const uint2 id = q.id[at];
+    const uint32_t k = id.y & 0x7FFFFFFu;
+    const double lm = lm_of((uint64_t)tc.q * k, tc.l4_d);
+    const double ax2 = 2.0 * __builtin_fabs(xy.x), ay2 = 2.0 * __builtin_fabs(xy.y);
+    const uint32_t terms = in ? id.y >> 28 : 0u;
+    bool pass = in & (terms == 0u);
+    for (uint32_t fl = terms; __builtin_amdgcn_ballot_w64(fl != 0u); fl &= fl - 1u) {
+        const uint32_t t = (uint32_t)__builtin_ctz(fl | 16u);  // 4 on lanes without a term left
+        const double a = t < 2u ? ax2 : ay2;
+        pass |= (fl != 0u) & root_near_integer((t & 1u) ? lm + a : lm - a);
+    }
+    if (__builtin_amdgcn_ballot_w64(pass)) {
+        const uint32_t ui = id.x & MAX_LEFT_ENTRIES, jj = id.x >> ID_U_BITS;
+        const uint32_t vidx = pass ? P.off[k] + jj : 0u;
+        const uint32_t kflags = k | ((id.y >> 27) & 1u) << 30 | (terms == 0u ? 0x80000000u : 0u);
+        push_global(P, pass, DefItem{tc.left, ui, vidx, kflags});
+    }
 }
 
-__device__ inline void root_push(const CellParams& P, RootQueue& q, uint32_t left, bool pred, double zz,
-                                 uint32_t ui, uint32_t vidx, uint32_t kc) {
-    uint64_t mask = __ballot(pred);
+__device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCtx& tc, bool pred, double X, double Y,
+                                 uint32_t idlo, uint32_t idhi) {
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
     // lanes without an item write the dummy slot QCAP-1 (never a live entry:
     // the queue holds at most 63 + 64 = 127 items), so the store is unconditional
     const uint32_t slot = pred ? q.n + lane_rank(mask) : (uint32_t)(QCAP - 1);
-    q.z[slot] = zz;
-    q.id[slot] = make_uint4(ui, vidx, kc, 0);
+    q.xy[slot] = make_double2(X, Y);
+    q.id[slot] = make_uint2(idlo, idhi);
     q.n += (uint32_t)__popcll(mask);
     if (q.n >= 64) {
         __builtin_amdgcn_wave_barrier();
-        root_drain(P, q, left, 64);
+        root_drain(P, q, tc, 64);
         __builtin_amdgcn_wave_barrier();
         const uint32_t lane = __lane_id();
-        double mz = q.z[64 + lane];
-        uint4 mi = q.id[64 + lane];
+        const double2 mxy = q.xy[64 + lane];
+        const uint2 mid = q.id[64 + lane];
         __builtin_amdgcn_wave_barrier();
         const uint32_t to = lane + 64 < q.n ? lane : (uint32_t)(QCAP - 1);
-        q.z[to] = mz;
-        q.id[to] = mi;
+        q.xy[to] = mxy;
+        q.id[to] = mid;
         __builtin_amdgcn_wave_barrier();
         q.n -= 64;
     }
 }
 
-// One leaf z/2 = X + iY of the fast path: the arithmetic of eval_leaf, with the
-// second orientation evaluated only when some lane's area test for it passes
-// (for l, m both odd it never does: Re z is odd, so 2 Re z != 0 mod 8).  Adds
-// the decided count to cand, sends an undecided leaf to the exact kernel and
-// queues counted terms for the root-proximity test.  kc = k | conj << 30.
-__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, uint32_t left, bool live, double X, double Y,
-                                 uint32_t vX, uint32_t vY, const LaneCtx& c, uint32_t ui, uint32_t vidx, uint32_t kc,
-                                 uint64_t& cand) {
-    const double ax = X < 0 ? -X : X, ay = Y < 0 ? -Y : Y;
-    const bool a1 = live && area_ok(vY, c);
-    bool t1, t2, u1;
-    range_terms(X, c, t1, t2, u1);
-    bool und = a1 && (ay <= c.tau || u1);
-    bool f0 = a1 && t1, f1 = a1 && t2, f2 = false, f3 = false;
-    const bool a2 = live && area_ok(vX, c);
-    if (__ballot(a2)) {
-        const double dm = X - Y, dp = X + Y;
-        const bool eq = (dm < 0 ? -dm : dm) <= c.tau || (dp < 0 ? -dp : dp) <= c.tau;
-        bool t3, t4, u2;
-        range_terms(Y, c, t3, t4, u2);
-        const bool o2 = a2 && !eq;
-        und = und || (a2 && (eq || ax <= c.tau)) || (o2 && u2);
-        f2 = o2 && t3;
-        f3 = o2 && t4;
+__device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
+    if (q.n) {
+        __builtin_amdgcn_wave_barrier();
+        root_drain(P, q, tc, q.n);
+        q.n = 0;
+        __builtin_amdgcn_wave_barrier();
     }
-    cand += und ? 0u : (uint32_t)f0 + (uint32_t)f1 + (uint32_t)f2 + (uint32_t)f3;
-    push_global(P, und, DefItem{left, ui, vidx, kc | 0x80000000u});
-    const bool ok = !und;
-    root_push(P, rq, left, ok && f0, c.lm - 2.0 * ax, ui, vidx, kc);
-    root_push(P, rq, left, ok && f1, c.lm + 2.0 * ax, ui, vidx, kc);
-    root_push(P, rq, left, ok && f2, c.lm - 2.0 * ay, ui, vidx, kc);
-    root_push(P, rq, left, ok && f3, c.lm + 2.0 * ay, ui, vidx, kc);
 }
 
-__global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
-    __shared__ double s_qz[WAVES][QCAP];
-    __shared__ uint4 s_qid[WAVES][QCAP];
+// One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
+// for lane data, with area residues pre-shifted (vXs = vX << 8).  Adds the
+// decided count to cnt and queues the leaf when it has counted terms (for the
+// root-proximity test) or is undecided (for the exact count).  kq = k | conj
+// << 27; idlo = pack_ui(ui, partner entry).
+__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
+                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
+                                 uint32_t& cnt) {
+    // bitwise logic on lane masks: no short-circuit branches
+    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
+    const bool a1 = live & area_ok_s(vYs, c);
+    const bool a2 = live & area_ok_s(vXs, c);
+    // orientation 1 (d = X) and 2 (d = Y): narrowed ranges decide, the widened
+    // ones minus the narrowed ones are the undecided band
+    const bool t1w = (ax >= c.h1_lo_w) & (ax <= c.h1_hi_w), t1n = (ax >= c.h1_lo_n) & (ax <= c.h1_hi_n);
+    const bool t2w = ax <= c.h2_w, t2n = ax <= c.h2_n;
+    const bool t3w = (ay >= c.h1_lo_w) & (ay <= c.h1_hi_w), t3n = (ay >= c.h1_lo_n) & (ay <= c.h1_hi_n);
+    const bool t4w = ay <= c.h2_w, t4n = ay <= c.h2_n;
+    const double dd = ax - ay;
+    const bool eq = __builtin_fabs(dd) <= c.tau;
+    const bool und = (a1 & ((t1w & !t1n) | (t2w & !t2n) | (ay <= c.tau))) |
+                     (a2 & ((t3w & !t3n) | (t4w & !t4n) | eq | (ax <= c.tau)));
+    const bool ok = !und;
+    const bool o2 = a2 & !eq & ok;
+    const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
+    cnt += (uint32_t)f0;
+    cnt += (uint32_t)f1;
+    cnt += (uint32_t)f2;
+    cnt += (uint32_t)f3;
+    const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
+    root_push(P, rq, tc, und | (terms != 0u), X, Y, idlo, kq | terms << 28);
+}
+
+// The per-pair context and residues of one partner entry v (lane data).
+struct Partner {
+    double c, d;
+    uint32_t c180s, d180s;  // (c mod 180) << 8, (d mod 180) << 8
+    uint32_t vidx, kj, jj;
+    bool vc;                // v is the central (real) entry of its half-set
+    LaneCtx ctx;
+};
+
+__device__ inline void load_partner(const CellParams& P, const LeftDesc& L, const PosDesc& S, bool act, uint32_t kj,
+                                    uint32_t jj, uint32_t Hj, uint32_t kdef, Partner& v) {
+    const uint32_t kjq = act ? kj : kdef;  // in-bounds index for every lane (act: kj <= k1)
+    v.kj = kj;
+    v.jj = jj;
+    v.vidx = act ? P.off[kjq] + jj : 0u;
+    v.vc = jj + 1 == Hj;
+    v.c = P.cre[v.vidx];
+    v.d = P.cim[v.vidx];
+    const uint32_t vr = P.cres[v.vidx];
+    v.c180s = (vr & 0xFFFFu) << 8;
+    v.d180s = (vr >> 8) & 0xFFFF00u;
+    const uint64_t m = (uint64_t)L.q * kj;
+    const uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
+    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, v.ctx);
+}
+
+// The central (real) entry u of the left set against v: leaf u*v only; with a
+// real v too, the real-leaf rule.
+__device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
+                                    const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
+    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+    const double t1 = u.im * v.d, t2 = u.re * v.d;
+    const uint32_t vXs = __umul24(ua, v.c180s) + (QA * QA << 8) - __umul24(ub, v.d180s);
+    const uint32_t vYs = __umul24(ua, v.d180s) + __umul24(ub, v.c180s);
+    const bool real = act && v.vc;
+    const bool a2 = real && area_ok_s(vXs, v.ctx);
+    // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
+    cnt += (a2 && v.ctx.h2_n >= 0.0) ? 1u : 0u;
+    push_global(P, a2 && v.ctx.h2_w >= 0.0 && v.ctx.h2_n < 0.0, DefItem{tc.left, ui, v.vidx, v.kj | 0x80000000u});
+    leaf_fast(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), vXs, vYs, v.ctx,
+              v.kj, pack_ui(ui, v.jj), cnt);
+}
+
+// Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
+// (k = kb + 2 lane) are filtered and their half-set entries flattened; f is
+// called for every 64 flattened entries with the lane's entry (act, kj, jj, Hj).
+template <typename F>
+__device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T, const LeftDesc& L, const PosDesc& S,
+                                          uint32_t* mark, uint64_t& pairs, F&& f) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t par = 0; par < 2; ++par) {
+        const uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
+        for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
+            const uint32_t k = kb + 2u * lane;
+            const uint32_t kq = k <= T.k1 ? k : T.k1;  // in-bounds index for every lane
+            bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
+            for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
+            for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
+            uint32_t Hk = P.off[kq + 1] - P.off[kq];
+            Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
+            pairs += Hk ? 1 : 0;
+            const uint32_t incl = wave_incl_scan(Hk);
+            const uint32_t excl = incl - Hk;
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            uint32_t carry = 0;
+            for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                const uint32_t j = step_owner(mark, excl, Hk, e0, carry);
+                const uint32_t Hj = (uint32_t)__shfl((int)Hk, (int)j, 64);
+                const uint32_t xj = (uint32_t)__shfl((int)excl, (int)j, 64);
+                const uint32_t jj = e - xj;
+                // an entry beyond the owner's run never happens; the test keeps
+                // every index in bounds regardless
+                const bool act = e < total && jj < Hj;
+                f(act, kb + 2u * j, act ? jj : 0u, Hj);
+            }
+        }
+    }
+}
+
+// Tile bookkeeping shared by both cell kernels: per-position counters.
+__device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t cnt, uint64_t leaves, uint64_t pairs,
+                                 uint64_t evald, unsigned long long* evaluated) {
+    const uint32_t lane = __lane_id();
+    const uint64_t cand = wave_sum64((uint64_t)cnt);
+    leaves = wave_sum64(leaves);
+    pairs = wave_sum64(pairs);
+    evald = wave_sum64(evald);
+    // all lanes execute the atomics; only lane 0 contributes
+    atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
+    atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
+    atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
+    atomicAdd(evaluated, lane == 0 ? (unsigned long long)evald : 0ull);
+}
+
+// The batch arrays the cell kernels only read, as separate restrict-qualified
+// arguments: uniform reads of them (tiles, left sets, positions, the left
+// entries walked by a whole wave) become scalar loads.
+#define HSS_CELL_ARGS                                                                                   \
+    const CellParams P, const BatchHeader* __restrict__ hdr, const TileDesc* __restrict__ tiles,       \
+        const LeftDesc* __restrict__ lefts, const PosDesc* __restrict__ pos,                            \
+        const LeftFast* __restrict__ lfast
+
+__global__ __launch_bounds__(WG, 5) void cell_kernel(HSS_CELL_ARGS) {
+    __shared__ double2 s_qxy[WAVES][QCAP];
+    __shared__ uint2 s_qid[WAVES][QCAP];
+    __shared__ uint32_t s_mark[WAVES][65];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_qz[wave], s_qid[wave], 0};
+    RootQueue rq{s_qxy[wave], s_qid[wave], 0};
+    uint32_t* mark = s_mark[wave];
     for (;;) {
         // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_tile, lane == 0 ? 1u : 0u));
-        if (t >= P.hdr->ntiles) break;
-        const TileDesc T = P.tiles[t];
-        const LeftDesc L = P.lefts[T.left];
-        const PosDesc& S = P.pos[L.pos];
-        uint64_t cand = 0, leaves = 0, pairs = 0;
-        for (uint32_t par = 0; par < 2; ++par) {
-            uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
-            for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
-                const uint32_t k = kb + 2u * (uint32_t)lane;
-                const uint32_t kq = k <= T.k1 ? k : T.k1;  // in-bounds index for every lane
-                bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
-                for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
-                for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
-                uint32_t Hk = P.off[kq + 1] - P.off[kq];
-                Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
-                pairs += Hk ? 1 : 0;
-                uint32_t incl = wave_incl_scan(Hk, lane);
-                uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
-                for (uint32_t e0 = 0; e0 < total; e0 += 64) {
-                    uint32_t e = e0 + (uint32_t)lane;
-                    bool act = e < total;
-                    uint32_t j = 0;
-                    for (uint32_t s = 32; s > 0; s >>= 1) {
-                        uint32_t v = __shfl(incl, (int)(j + s - 1), 64);
-                        if (v <= e) j += s;
-                    }
-                    j = j > 63 ? 63 : j;
-                    uint32_t inclj = __shfl(incl, (int)j, 64);
-                    uint32_t Hj = __shfl(Hk, (int)j, 64);
-                    const uint32_t kj = kb + 2u * j;
-                    const uint32_t jj = e - (inclj - Hj);
-                    const uint32_t kjq = act ? kj : T.k0;  // in-bounds index for every lane
-                    const uint32_t vidx = act ? P.off[kjq] + jj : 0u;
-                    bool vc = jj + 1 == Hj;
-                    double c = P.cre[vidx], d = P.cim[vidx];
-                    uint32_t vr = P.cres[vidx];
-                    uint32_t c180 = vr & 0xFFFFu, d180 = vr >> 16;
-                    uint64_t m = (uint64_t)L.q * kj;
-                    uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
-                    LaneCtx ctx;
-                    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, ctx);
-                    leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
-                    // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
-                    for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
-                        const LeftFast u = P.lfast[L.off + ui];
-                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                        const double t1 = u.im * d, t2 = u.re * d;
-                        const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
-                        leaf_fast(P, rq, T.left, act, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
-                                  pac + QA * QA - pbd, pad + pbc, ctx, ui, vidx, kj, cand);
-                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, t1), __builtin_fma(u.im, c, -t2),
-                                  pac + pbd, pbc + QA * QA - pad, ctx, ui, vidx, kj | 1u << 30, cand);
-                    }
-                    // the central (real) u: leaf u*v only; with a real v too the real-leaf rule
-                    {
-                        const uint32_t ui = L.H - 1;
-                        const LeftFast u = P.lfast[L.off + ui];
-                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                        const double t1 = u.im * d, t2 = u.re * d;
-                        const uint32_t vX = ua * c180 + QA * QA - ub * d180, vY = ua * d180 + ub * c180;
-                        LeafEval rr;
-                        eval_real_leaf(vX, ctx, rr);
-                        const bool real = act && vc;
-                        cand += (real && !rr.und) ? rr.cnt : 0u;
-                        push_global(P, real && rr.und, DefItem{T.left, ui, vidx, kj | 0x80000000u});
-                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
-                                  vX, vY, ctx, ui, vidx, kj, cand);
-                    }
-                }
+        if (t >= hdr->ntiles) break;
+        const TileDesc T = tiles[t];
+        const LeftDesc L = lefts[T.left];
+        const PosDesc& S = pos[L.pos];
+        const TileCtx tc{T.left, L.q, S.l4_d};
+        const LeftFast* lf = lfast + L.off;
+        uint32_t cnt = 0;
+        uint64_t leaves = 0, pairs = 0;
+        for_partner_chunks(P, T, L, S, mark, pairs, [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj) {
+            Partner v;
+            load_partner(P, L, S, act, kj, jj, Hj, T.k0, v);
+            leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
+            const uint32_t jpk = jj << ID_U_BITS;
+            // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
+            for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
+                const LeftFast u = lf[ui];
+                const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                const double t1 = u.im * v.d, t2 = u.re * v.d;
+                const uint32_t pac = __umul24(ua, v.c180s), pbd = __umul24(ub, v.d180s);
+                const uint32_t pad = __umul24(ua, v.d180s), pbc = __umul24(ub, v.c180s);
+                leaf_fast(P, rq, tc, act, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2),
+                          pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, kj, ui | jpk, cnt);
+                leaf_fast(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, t1), __builtin_fma(u.im, v.c, -t2),
+                          pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, kj | 1u << 27, ui | jpk, cnt);
             }
-        }
-        if (rq.n) {
-            __builtin_amdgcn_wave_barrier();
-            root_drain(P, rq, T.left, rq.n);
-            rq.n = 0;
-            __builtin_amdgcn_wave_barrier();
-        }
-        cand = wave_sum64(cand);
-        leaves = wave_sum64(leaves);
-        pairs = wave_sum64(pairs);
-        // all lanes execute the atomics; only lane 0 contributes
-        atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
-        atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
-        atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
-        atomicAdd(P.evaluated + 1, lane == 0 ? (unsigned long long)leaves : 0ull);  // every leaf is evaluated
+            central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
+        });
+        root_flush(P, rq, tc);
+        tile_done(P, S, cnt, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
     }
 }
 
@@ -551,195 +682,129 @@ __global__ __launch_bounds__(WG) void cell_kernel(CellParams P) {
 //     with leaf_fast from its owner's record and a vector load of u.
 // Leaves outside the ranges count 0 exactly and are skipped.
 
-// 112 bytes: four blocks of four waves fit a CU's LDS with the root queues.
-// The narrowed range bounds are derived from the widened ones (n = w -+ 2 tau).
+// 104 bytes: four blocks of four waves fit a CU's LDS with the root queues.
+// The narrowed range bounds are derived from the widened ones (n = w -+ tau).
 struct WinRec {
     double c, d;  // partner v
-    double t1_lo_w, t1_hi_w, t2_w, tau, lm;
-    uint32_t inv_h, thr_h;
-    uint32_t cres;   // (c mod 180) | (d mod 180) << 16
-    uint32_t vidx;
+    double h1_lo_w, h1_hi_w, h2_w, tau;
+    uint32_t inv24, thr22;
+    uint32_t cres;   // (c mod 180) << 8 | (d mod 180) << 24
+    uint32_t first;  // the partner's first item in the chunk's flattened items
     uint32_t kj;
+    uint32_t jpk;     // partner entry << ID_U_BITS
     uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
     uint16_t cum[8];  // items before each range
 };
-static_assert(sizeof(WinRec) == 112, "LDS budget of cell_win_kernel");
+static_assert(sizeof(WinRec) == 104, "LDS budget of cell_win_kernel");
 
-__global__ __launch_bounds__(WG) void cell_win_kernel(CellParams P) {
-    __shared__ double s_qz[WAVES][QCAP];
-    __shared__ uint4 s_qid[WAVES][QCAP];
+__global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
+    __shared__ double2 s_qxy[WAVES][QCAP];
+    __shared__ uint2 s_qid[WAVES][QCAP];
     __shared__ WinRec s_rec[WAVES][64];
+    __shared__ uint32_t s_mark[WAVES][65];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_qz[wave], s_qid[wave], 0};
+    RootQueue rq{s_qxy[wave], s_qid[wave], 0};
     WinRec* rec = s_rec[wave];
+    uint32_t* mark = s_mark[wave];
     for (;;) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_wtile, lane == 0 ? 1u : 0u));
-        if (t >= P.hdr->nwtiles) break;
-        const TileDesc T = P.wtiles[t];
-        const LeftDesc L = P.lefts[T.left];
-        const PosDesc& S = P.pos[L.pos];
+        if (t >= hdr->nwtiles) break;
+        const TileDesc T = tiles[t];
+        const LeftDesc L = lefts[T.left];
+        const PosDesc& S = pos[L.pos];
+        const TileCtx tc{T.left, L.q, S.l4_d};
+        const LeftFast* lf = lfast + L.off;
         const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
         const uint32_t nsort = L.H - 1;
-        uint64_t cand = 0, leaves = 0, pairs = 0, evald = 0;
-        for (uint32_t par = 0; par < 2; ++par) {
-            uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
-            for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
-                const uint32_t k = kb + 2u * (uint32_t)lane;
-                const uint32_t kq = k <= T.k1 ? k : T.k1;
-                bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
-                for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
-                for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
-                uint32_t Hk = P.off[kq + 1] - P.off[kq];
-                Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
-                pairs += Hk ? 1 : 0;
-                uint32_t incl = wave_incl_scan(Hk, lane);
-                uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
-                for (uint32_t e0 = 0; e0 < total; e0 += 64) {
-                    // ---- phase 1: lane = partner entry
-                    uint32_t e = e0 + (uint32_t)lane;
-                    bool act = e < total;
-                    uint32_t j = 0;
-                    for (uint32_t s = 32; s > 0; s >>= 1) {
-                        uint32_t v = __shfl(incl, (int)(j + s - 1), 64);
-                        if (v <= e) j += s;
-                    }
-                    j = j > 63 ? 63 : j;
-                    uint32_t inclj = __shfl(incl, (int)j, 64);
-                    uint32_t Hj = __shfl(Hk, (int)j, 64);
-                    const uint32_t kj = kb + 2u * j;
-                    const uint32_t jj = e - (inclj - Hj);
-                    const uint32_t kjq = act ? kj : T.k0;
-                    const uint32_t vidx = act ? P.off[kjq] + jj : 0u;
-                    const bool vc = jj + 1 == Hj;
-                    const double c = P.cre[vidx], d = P.cim[vidx];
-                    const uint32_t vr = P.cres[vidx];
-                    const uint32_t c180 = vr & 0xFFFFu, d180 = vr >> 16;
-                    const uint64_t m = (uint64_t)L.q * kj;
-                    const uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
-                    LaneCtx ctx;
-                    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, ctx);
-                    leaves += act ? (vc ? L.H : 2ull * L.H - 1) : 0ull;
-                    {  // the central (real) u, as in cell_kernel
-                        const uint32_t ui = L.H - 1;
-                        const LeftFast u = P.lfast[L.off + ui];
-                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                        const double t1 = u.im * d, t2 = u.re * d;
-                        const uint32_t vX = ua * c180 + QA * QA - ub * d180, vY = ua * d180 + ub * c180;
-                        LeafEval rr;
-                        eval_real_leaf(vX, ctx, rr);
-                        const bool real = act && vc;
-                        cand += (real && !rr.und) ? rr.cnt : 0u;
-                        push_global(P, real && rr.und, DefItem{T.left, ui, vidx, kj | 0x80000000u});
-                        leaf_fast(P, rq, T.left, act && !vc, __builtin_fma(u.re, c, -t1), __builtin_fma(u.im, c, t2),
-                                  vX, vY, ctx, ui, vidx, kj, cand);
-                    }
-                    // ranges of non-central entries
-                    const double m2 = (double)(m * m);
-                    Window w;
-                    make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, ctx.tau, 2.0 * S.l2_d * m2, w);
-                    const double pv = psi_of_f(c, d);
-                    uint32_t rs[8], re[8];
-                    uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
-                    uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
-                    n0 = act ? n0 : 0u;
-                    n1 = (act && !vc) ? n1 : 0u;
-                    const uint32_t cnt = n0 + n1;
-                    evald += act ? 1u + cnt : 0u;  // the central leaf and the items
-                    __builtin_amdgcn_wave_barrier();
-                    WinRec& R = rec[lane];
-                    R.c = c;
-                    R.d = d;
-                    R.t1_lo_w = ctx.t1_lo_w;
-                    R.t1_hi_w = ctx.t1_hi_w;
-                    R.t2_w = ctx.t2_w;
-                    R.tau = ctx.tau;
-                    R.lm = ctx.lm;
-                    R.inv_h = ctx.inv_h;
-                    R.thr_h = ctx.thr_h;
-                    R.cres = vr;
-                    R.vidx = vidx;
-                    R.kj = kj;
-                    uint32_t acc = 0;
-                    for (int p = 0; p < 8; ++p) {
-                        const bool live = p < 4 ? n0 != 0 : n1 != 0;
-                        const uint32_t len = live ? re[p] - rs[p] : 0u;
-                        R.st[p] = (uint16_t)rs[p];
-                        R.cum[p] = (uint16_t)acc;
-                        acc += len;
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    // ---- phase 2: lane = item
-                    const uint32_t icl = wave_incl_scan(cnt, lane);
-                    const uint32_t excl = icl - cnt;  // first item of this lane's partner
-                    const uint32_t items = __builtin_amdgcn_readfirstlane(__shfl(icl, 63, 64));
-                    // Owners' items are contiguous and in lane order, so a step of 64
-                    // items spans few owners: a uniform cursor walks the owners whose
-                    // items start inside the step (scalar reads of their starts).
-                    uint32_t cur = 0;  // an owner whose items start at or before the step
-                    for (uint32_t i0 = 0; i0 < items; i0 += 64) {
-                        const uint32_t it = i0 + (uint32_t)lane;
-                        const bool live = it < items;
-                        cur = __builtin_amdgcn_readfirstlane(cur);  // uniform: keep it scalar
-                        uint32_t o = cur, so = __builtin_amdgcn_readlane(excl, cur);
-                        uint32_t nx = cur + 1;
-                        for (; nx < 64; ++nx) {
-                            const uint32_t snx = __builtin_amdgcn_readlane(excl, nx);
-                            if (snx >= i0 + 64) break;
-                            const bool past = it >= snx;
-                            o = past ? nx : o;
-                            so = past ? snx : so;
-                        }
-                        cur = nx - 1;
-                        const uint32_t q = live ? it - so : 0u;
-                        const WinRec& Q = rec[o];
-                        uint32_t pc = 0;
-                        for (int p = 1; p < 8; ++p) pc += Q.cum[p] <= q ? 1u : 0u;
-                        const uint32_t ui0 = (uint32_t)Q.st[pc] + (q - (uint32_t)Q.cum[pc]);
-                        const uint32_t ui = live && ui0 < nsort ? ui0 : 0u;  // in-bounds for every lane
-                        const bool cj = pc >= 4;
-                        const LeftFast u = P.lfast[L.off + ui];
-                        const double qc = Q.c, qd = Q.d;
-                        const uint32_t qr = Q.cres;
-                        const uint32_t qc180 = qr & 0xFFFFu, qd180 = qr >> 16;
-                        LaneCtx qx;
-                        qx.t1_lo_w = Q.t1_lo_w;
-                        qx.t1_hi_w = Q.t1_hi_w;
-                        qx.t2_w = Q.t2_w;
-                        qx.tau = Q.tau;
-                        qx.t1_lo_n = __builtin_fma(2.0, Q.tau, Q.t1_lo_w);
-                        qx.t1_hi_n = __builtin_fma(-2.0, Q.tau, Q.t1_hi_w);
-                        qx.t2_n = __builtin_fma(-2.0, Q.tau, Q.t2_w);
-                        qx.lm = Q.lm;
-                        qx.inv_h = Q.inv_h;
-                        qx.thr_h = Q.thr_h;
-                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                        const double t1 = u.im * qd, t2 = u.re * qd;
-                        const uint32_t pac = ua * qc180, pbd = ub * qd180, pad = ua * qd180, pbc = ub * qc180;
-                        const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
-                        const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
-                        const uint32_t vX = cj ? pac + pbd : pac + QA * QA - pbd;
-                        const uint32_t vY = cj ? pbc + QA * QA - pad : pad + pbc;
-                        leaf_fast(P, rq, T.left, live, X, Y, vX, vY, qx, ui, Q.vidx, Q.kj | (cj ? 1u << 30 : 0u),
-                                  cand);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                }
+        uint32_t cnt = 0;
+        uint64_t leaves = 0, pairs = 0, evald = 0;
+        for_partner_chunks(P, T, L, S, mark, pairs, [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj) {
+            // ---- phase 1: lane = partner entry
+            Partner v;
+            load_partner(P, L, S, act, kj, jj, Hj, T.k0, v);
+            leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
+            central_leaf(P, rq, tc, act, lf[nsort], nsort, v, cnt);
+            // ranges of non-central entries
+            const uint64_t m = (uint64_t)L.q * kj;
+            const double m2 = (double)(m * m);
+            Window w;
+            make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, v.ctx.tau, 2.0 * S.l2_d * m2, w);
+            const double pv = psi_of_f(v.c, v.d);
+            uint32_t rs[8], re[8];
+            uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
+            uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
+            n0 = act ? n0 : 0u;
+            n1 = (act && !v.vc) ? n1 : 0u;
+            const uint32_t cntv = n0 + n1;
+            evald += act ? 1u + cntv : 0u;  // the central leaf and the items
+            const uint32_t icl = wave_incl_scan(cntv);
+            const uint32_t excl = icl - cntv;  // first item of this lane's partner
+            const uint32_t items = __builtin_amdgcn_readlane(icl, 63);
+            __builtin_amdgcn_wave_barrier();
+            WinRec& R = rec[lane];
+            R.c = v.c;
+            R.d = v.d;
+            R.h1_lo_w = v.ctx.h1_lo_w;
+            R.h1_hi_w = v.ctx.h1_hi_w;
+            R.h2_w = v.ctx.h2_w;
+            R.tau = v.ctx.tau;
+            R.inv24 = v.ctx.inv24;
+            R.thr22 = v.ctx.thr22;
+            R.cres = v.c180s | v.d180s << 16;
+            R.first = excl;
+            R.kj = kj;
+            R.jpk = jj << ID_U_BITS;
+            uint32_t acc = 0;
+            for (int p = 0; p < 8; ++p) {
+                const bool live = p < 4 ? n0 != 0 : n1 != 0;
+                const uint32_t len = live ? re[p] - rs[p] : 0u;
+                R.st[p] = (uint16_t)rs[p];
+                R.cum[p] = (uint16_t)acc;
+                acc += len;
             }
-        }
-        if (rq.n) {
             __builtin_amdgcn_wave_barrier();
-            root_drain(P, rq, T.left, rq.n);
-            rq.n = 0;
+            // ---- phase 2: lane = item
+            uint32_t carry = 0;
+            for (uint32_t i0 = 0; i0 < items; i0 += 64) {
+                const uint32_t it = i0 + (uint32_t)lane;
+                const uint32_t o = step_owner(mark, excl, cntv, i0, carry);
+                const WinRec& Q = rec[o];
+                const bool live = it < items;
+                const uint32_t q = live ? it - Q.first : 0u;
+                uint32_t pc = 0;
+                for (int p = 1; p < 8; ++p) pc += Q.cum[p] <= q ? 1u : 0u;
+                const uint32_t ui0 = (uint32_t)Q.st[pc] + (q - (uint32_t)Q.cum[pc]);
+                const uint32_t ui = live && ui0 < nsort ? ui0 : 0u;  // in-bounds for every lane
+                const bool cj = pc >= 4;
+                const LeftFast u = lf[ui];
+                const double qc = Q.c, qd = Q.d;
+                const uint32_t qr = Q.cres;
+                const uint32_t qc180s = qr & 0xFFFFu, qd180s = qr >> 16;
+                LaneCtx qx;
+                qx.h1_lo_w = Q.h1_lo_w;
+                qx.h1_hi_w = Q.h1_hi_w;
+                qx.h2_w = Q.h2_w;
+                qx.tau = Q.tau;
+                qx.h1_lo_n = Q.h1_lo_w + Q.tau;
+                qx.h1_hi_n = Q.h1_hi_w - Q.tau;
+                qx.h2_n = Q.h2_w - Q.tau;
+                qx.inv24 = Q.inv24;
+                qx.thr22 = Q.thr22;
+                const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                const double t1 = u.im * qd, t2 = u.re * qd;
+                const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
+                const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
+                const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
+                const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
+                const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
+                const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
+                const uint32_t kq = Q.kj | (cj ? 1u << 27 : 0u);
+                leaf_fast(P, rq, tc, live, X, Y, vXs, vYs, qx, kq, ui | Q.jpk, cnt);
+            }
             __builtin_amdgcn_wave_barrier();
-        }
-        cand = wave_sum64(cand);
-        leaves = wave_sum64(leaves);
-        pairs = wave_sum64(pairs);
-        evald = wave_sum64(evald);
-        atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
-        atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
-        atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
-        atomicAdd(P.evaluated, lane == 0 ? (unsigned long long)evald : 0ull);
+        });
+        root_flush(P, rq, tc);
+        tile_done(P, S, cnt, leaves, pairs, evald, P.evaluated);
     }
 }
 
@@ -865,12 +930,12 @@ hipError_t launch_build_left(const BatchHeader* hdr, const LeftBuild* builds, co
 }
 
 hipError_t launch_cells(const CellParams& P, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL(cell_kernel, dim3(blocks), dim3(WG), 0, st, P);
+    hipLaunchKernelGGL(cell_kernel, dim3(blocks), dim3(WG), 0, st, P, P.hdr, P.tiles, P.lefts, P.pos, P.lfast);
     return hipGetLastError();
 }
 
 hipError_t launch_cells_win(const CellParams& P, uint32_t blocks, hipStream_t st) {
-    hipLaunchKernelGGL(cell_win_kernel, dim3(blocks), dim3(WG), 0, st, P);
+    hipLaunchKernelGGL(cell_win_kernel, dim3(blocks), dim3(WG), 0, st, P, P.hdr, P.wtiles, P.lefts, P.pos, P.lfast);
     return hipGetLastError();
 }
 

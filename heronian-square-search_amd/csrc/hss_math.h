@@ -101,69 +101,87 @@ HSS_HD inline bool bit_of(const uint32_t* words, uint32_t i) { return (words[i >
 // ------------------------------------------------- per-(l, m) fast-path state
 
 struct LaneCtx {
-    // range bounds on |2D| widened (w) and narrowed (n) by tau:
+    // range bounds on |D| (halved bounds on |2D|: halving is exact), widened (w)
+    // and narrowed (n) by tau / 2:
     //   term1: lo+ <= |2D| <= hi (d = +|2D|),  term2: |2D| <= -lo (d = -|2D|)
-    double t1_lo_w, t1_lo_n, t1_hi_w, t1_hi_n;
-    double t2_w, t2_n;
+    double h1_lo_w, h1_lo_n, h1_hi_w, h1_hi_n;
+    double h2_w, h2_n;
     double tau;   // undecided band: FP64 errors stay below tau / 16
     double lm;    // FP64 value of l^4 + m^4
-    uint32_t inv_h;  // inverse of h = (3 or 9) * (1 or 5) modulo 2^24
-    uint32_t thr_h;  // floor((2^24 - 1) / h)
+    uint32_t inv24;  // inverse of h = (3 or 9) * (1 or 5) modulo 2^24
+    uint32_t thr22;  // floor((2^22 - 1) / h)
 };
 
-HSS_HD inline uint32_t inv_mod_2_24(uint32_t h) {
+HSS_HD constexpr uint32_t inv_mod_2_24(uint32_t h) {
     uint32_t x = h;  // Newton iteration for the inverse of an odd h modulo 2^24
     for (int i = 0; i < 5; ++i) x *= 2u - h * x;
     return x & 0xFFFFFFu;
+}
+
+// FP64 value of l^4 + m^4 (the zz estimates of a pair are lm -+ 2|D|)
+HSS_HD inline double lm_of(uint64_t m, double l4_d) {
+    const double md = (double)(uint32_t)m;  // exact: m < 2^26
+    const double m2 = md * md;
+    return l4_d + m2 * m2;
 }
 
 // Per-pair context.  l4_d, n4_d: FP64 values of l^4 and N^4; l2_d = l^2;
 // l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5.
 HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
                                  uint32_t m3, uint32_t m5, LaneCtx& c) {
-    double m2 = (double)(m * m);  // exact: m < 2^26
+    const double md = (double)(uint32_t)m;  // exact: m < 2^26
+    double m2 = md * md;                     // exact: m^2 < 2^52
     double m4 = m2 * m2;
     double lo = (l4_d + m4) - n4_d;
     double hi = l4_d;
     double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
     double lop = lo > 0.0 ? lo : 0.0;
-    c.t1_lo_w = lop - tau;
-    c.t1_lo_n = lop + tau;
-    c.t1_hi_w = hi + tau;
-    c.t1_hi_n = hi - tau;
-    c.t2_w = -lo + tau;
-    c.t2_n = -lo - tau;
+    c.h1_lo_w = (lop - tau) * 0.5;
+    c.h1_lo_n = (lop + tau) * 0.5;
+    c.h1_hi_w = (hi + tau) * 0.5;
+    c.h1_hi_n = (hi - tau) * 0.5;
+    c.h2_w = (-lo + tau) * 0.5;
+    c.h2_n = (-lo - tau) * 0.5;
     c.tau = tau;
     c.lm = l4_d + m4;
-    bool need9 = l3 == 0 || m3 == 0;
-    bool need5 = l5 != 0 && m5 != 0;
-    uint32_t h = (need9 ? 9u : 3u) * (need5 ? 5u : 1u);
-    c.inv_h = inv_mod_2_24(h);
-    c.thr_h = 0xFFFFFFu / h;
+    // h = (9 if 3 | l or 3 | m, else 3) * (5 if 5 divides neither, else 1)
+    const bool need9 = l3 == 0 || m3 == 0;
+    const bool need5 = l5 != 0 && m5 != 0;
+    constexpr uint32_t I3 = inv_mod_2_24(3), I9 = inv_mod_2_24(9), I15 = inv_mod_2_24(15), I45 = inv_mod_2_24(45);
+    c.inv24 = need9 ? (need5 ? I45 : I9) : (need5 ? I15 : I3);
+    c.thr22 = need9 ? (need5 ? 0x3FFFFFu / 45 : 0x3FFFFFu / 9) : (need5 ? 0x3FFFFFu / 15 : 0x3FFFFFu / 3);
 }
 
-// area4 = 2A: area4 = 0 (mod 8) and |area4| = 0 (mod h), from any va = A (mod 180)
-// with 0 <= va < 2^24
-HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) {
-    return (va & 3u) == 0 && ((va * c.inv_h) & 0xFFFFFFu) <= c.thr_h;
+// area4 = 2A: area4 = 0 (mod 8) and |area4| = 0 (mod h), from any va = A (mod
+// 180) with 0 <= va < 2^16, given as vs = va << 8.  With x = va * h^-1 mod 2^24,
+// va = 0 (mod 4h) iff the 24-bit rotation of x right by 2 is at most
+// (2^22 - 1) / h (divisibility by an odd h through its inverse, the factor 4
+// through the rotated-out low bits).  vs * inv24 mod 2^32 = x << 8, so the
+// rotation is a 32-bit rotation by 10 of that product.
+HSS_HD inline bool area_ok_s(uint32_t vs, const LaneCtx& c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t x;  // both operands are below 2^24: one 24-bit multiply, no masking
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(x) : "v"(vs), "v"(c.inv24));
+#else
+    const uint32_t x = vs * c.inv24;
+#endif
+    return ((x >> 10) | (x << 22)) <= c.thr22;
 }
+
+HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) { return area_ok_s(va << 8, c); }
 
 // Range terms of one orientation with d = 2D: term1 counts d = +|2D|, term2
 // d = -|2D|.  undecided is set when the FP64 estimate is within tau of a bound.
 HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, bool& undecided) {
-    double x = 2.0 * (Dd < 0 ? -Dd : Dd);
-    bool t1w = x >= c.t1_lo_w && x <= c.t1_hi_w;
-    bool t1n = x >= c.t1_lo_n && x <= c.t1_hi_n;
-    bool t2w = x <= c.t2_w;
-    bool t2n = x <= c.t2_n;
-    undecided = (t1w != t1n) || (t2w != t2n);
+    double x = Dd < 0 ? -Dd : Dd;
+    bool t1w = x >= c.h1_lo_w && x <= c.h1_hi_w;
+    bool t1n = x >= c.h1_lo_n && x <= c.h1_hi_n;
+    bool t2w = x <= c.h2_w;
+    bool t2n = x <= c.h2_n;
+    undecided = (t1w && !t1n) || (t2w && !t2n);  // the narrowed ranges lie inside the widened ones
     t1 = t1n;
     t2 = t2n;
 }
-
-// zz estimates of the two terms
-HSS_HD inline double zz_term1(double Dd, const LaneCtx& c) { return c.lm - 2.0 * (Dd < 0 ? -Dd : Dd); }
-HSS_HD inline double zz_term2(double Dd, const LaneCtx& c) { return c.lm + 2.0 * (Dd < 0 ? -Dd : Dd); }
 
 // can an FP64 estimate zz (relative error < 2^-49, zz >= 1) be an exact fourth power?
 HSS_HD inline bool root_near_integer(double zz) {
@@ -175,7 +193,9 @@ HSS_HD inline bool root_near_integer(double zz) {
 // One leaf z/2 = X + iY with area residues vX = X, vY = Y (mod 180).  cnt is
 // the decided candidate count; und is set when the leaf must be counted by the
 // exact stage instead; f[t] marks counted terms (orientation 1: term1, term2;
-// orientation 2: term1, term2) and zz[t] their FP64 zz estimates.
+// orientation 2: term1, term2) and zz[t] their FP64 zz estimates.  The two
+// orientations coincide in magnitude (eq) when min(|X - Y|, |X + Y|) =
+// ||X| - |Y|| (the same FP64 value either way) is within tau.
 struct LeafEval {
     uint32_t cnt;
     bool und;
@@ -187,14 +207,13 @@ HSS_HD inline void eval_leaf(double X, double Y, uint32_t vX, uint32_t vY, const
     bool a1 = area_ok(vY, c);
     bool a2 = area_ok(vX, c);
     double ax = X < 0 ? -X : X, ay = Y < 0 ? -Y : Y;
-    double dm = X - Y, dp = X + Y;
-    bool eq = (dm < 0 ? -dm : dm) <= c.tau || (dp < 0 ? -dp : dp) <= c.tau;
-    bool und = (a2 && eq) || (a1 && ay <= c.tau) || (a2 && ax <= c.tau);
+    double dd = ax - ay;
+    bool eq = (dd < 0 ? -dd : dd) <= c.tau;
     bool t1, t2, u1, t3, t4, u2;
     range_terms(X, c, t1, t2, u1);
     range_terms(Y, c, t3, t4, u2);
+    bool und = (a1 && (u1 || ay <= c.tau)) || (a2 && (u2 || eq || ax <= c.tau));
     bool o2 = a2 && !eq;
-    und = und || (a1 && u1) || (o2 && u2);
     r.f[0] = a1 && t1;
     r.f[1] = a1 && t2;
     r.f[2] = o2 && t3;
@@ -215,8 +234,8 @@ HSS_HD inline void eval_leaf(double X, double Y, uint32_t vX, uint32_t vY, const
 HSS_HD inline void eval_real_leaf(uint32_t vX, const LaneCtx& c, LeafEval& r) {
     bool a2 = area_ok(vX, c);
     // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
-    r.cnt = (a2 && c.t2_n >= 0.0) ? 1u : 0u;
-    r.und = a2 && c.t2_w >= 0.0 && c.t2_n < 0.0;
+    r.cnt = (a2 && c.h2_n >= 0.0) ? 1u : 0u;
+    r.und = a2 && c.h2_w >= 0.0 && c.h2_n < 0.0;
     r.f[0] = r.f[1] = r.f[2] = r.f[3] = false;
     r.zz[0] = r.zz[1] = r.zz[2] = r.zz[3] = 0.0;
 }

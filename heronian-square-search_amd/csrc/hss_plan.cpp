@@ -230,10 +230,15 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                     B.p[j] = f.sp[j];
                     B.E[j] = f.se[j] + sexp[j];
                 }
+                const uint64_t H = left_size(f, sexp);
+                if (H > MAX_LEFT_ENTRIES) {
+                    if (C.err.empty()) C.err = "left set larger than 2^17 - 1 entries";
+                    return;
+                }
                 LeftDesc L{};
                 L.pos = s;
                 L.off = (uint32_t)C.entries;  // relative to the chunk until merged
-                L.H = (uint32_t)left_size(f, sexp);
+                L.H = (uint32_t)H;
                 L.q = (uint32_t)q;
                 L.trivial = f.ns == 0 ? 1u : 0u;
                 L.q3 = (uint32_t)(q % 3);

@@ -190,6 +190,8 @@ static int build_tables(hss_ctx* c) {
     HIP_TRY(hipStreamSynchronize(st));
     uint64_t total = c->h_off[n + 1];
     if (total >= 0xFFFFFFFFull) return fail(HSS_EINVAL, "cache larger than 2^32 entries");
+    for (uint64_t k = 0; k <= n; ++k)
+        if (c->h_off[k + 1] - c->h_off[k] > MAX_HALF_SET) return fail(HSS_EINVAL, "half-set larger than 2^15 - 1");
     c->cache_total = (uint32_t)total;
     // split primes and their Gaussian value blocks
     uint64_t cap = n >= 5 ? (n - 5) / 4 + 1 : 1;

@@ -16,6 +16,13 @@ static constexpr uint32_t WIN_NB = 1024;
 static constexpr uint32_t WIN_MAX_SORT = 4096;
 static constexpr uint32_t WIN_MIN_SORT = 8;
 static constexpr uint32_t NO_TABLE = 0xFFFFFFFFu;
+// Leaves queued for the root test are named by (entry u, partner entry within
+// its half-set, k, conjugate, counted terms) packed into 64 bits: u below
+// 2^17, the partner entry below 2^15 (the planner and the table build check),
+// k below 2^27 (k <= N_max <= 2^26).
+static constexpr uint32_t ID_U_BITS = 17;
+static constexpr uint32_t MAX_LEFT_ENTRIES = (1u << ID_U_BITS) - 1;
+static constexpr uint32_t MAX_HALF_SET = (1u << (32 - ID_U_BITS)) - 1;
 
 // One root position l of a batch (uniform across a tile).  Positions of one
 // batch may carry different LIMITs.

@@ -317,6 +317,23 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
     return hc_search_mixed(nmax, roots, limits.data(), n_roots, cand, leaves, pairs, hits, hit_cap, n_hits, st);
 }
 
+// The fast path's area test (one 24-bit multiply and a rotation) against
+// divisibility by 4h, for every residue value the kernels form (below 2^16)
+// and every (l mod 3, l mod 5, m mod 3, m mod 5); returns the mismatches.
+uint64_t hc_area_mismatches(void) {
+    uint64_t bad = 0;
+    for (uint32_t l3 = 0; l3 < 3; ++l3)
+        for (uint32_t l5 = 0; l5 < 5; ++l5)
+            for (uint32_t m3 = 0; m3 < 3; ++m3)
+                for (uint32_t m5 = 0; m5 < 5; ++m5) {
+                    LaneCtx c;
+                    make_lane_ctx(7, 1.0, 1.0, 2.0, l3, l5, m3, m5, c);
+                    const uint32_t h = ((l3 == 0 || m3 == 0) ? 9u : 3u) * ((l5 != 0 && m5 != 0) ? 5u : 1u);
+                    for (uint32_t va = 0; va < 65536; ++va) bad += area_ok(va, c) != (va % (4 * h) == 0);
+                }
+    return bad;
+}
+
 // Mean range-window fraction of root l over partners q*k, k in [k0, k1].
 double hc_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1) {
     return tile_window_fraction(l, limit, q, k0, k1, 33);

@@ -199,6 +199,13 @@ def test_window_mode_skips_work_near_limit(hc, window_everywhere):
     assert sw.cells < 0.5 * sp.cells, (sw.cells, sp.cells)
 
 
+def test_area_test_is_divisibility(hc):
+    """The fast path's area test (24-bit inverse multiply, rotation) accepts
+    exactly the residues divisible by 4h, h = (3 or 9) * (1 or 5)."""
+    hc.hc_area_mismatches.restype = ctypes.c_uint64
+    assert hc.hc_area_mismatches() == 0
+
+
 def test_window_fraction_bounds(hc):
     """The planner's window measure is a fraction of the circle, 1 where the
     range test cannot fail (small m, large LIMIT) and small near LIMIT."""
