@@ -427,35 +427,49 @@ struct TileCtx {
 
 __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj << ID_U_BITS; }
 
-// The first `count` queued leaves, one per lane: an undecided leaf goes to the
-// deferred buffer for the full exact count; a leaf with counted terms goes
-// there when one of their zz estimates passes the root-proximity test.  The
-// wave tests one term per lane at a time, as many rounds as the most terms
-// any lane has.
-__device__ inline void root_drain(const CellParams& P, const RootQueue& q, const TileCtx& tc, uint32_t count) {
+// One round over the first min(n, 64) queued leaves, one per lane: an
+// undecided leaf goes to the deferred buffer for the full exact count; a leaf
+// with counted terms has its lowest remaining term root-tested (a survivor goes
+// to the deferred buffer) and is queued again while terms remain, so every
+// round tests a full wave of terms.  The entries after the first 64 move to
+// the front, the re-queued ones follow them.
+__device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
+    const uint32_t count = q.n < 64u ? q.n : 64u;
+    const uint32_t keep = q.n - count;
     const bool in = lane < count;
-    const uint32_t at = in ? lane : 0u;  // branch-free: every lane reads a valid slot
-    const double2 xy = q.xy[at];
-    const uint2 id = q.id[at];
+    __builtin_amdgcn_wave_barrier();
+    const double2 xy = q.xy[lane];
+    const uint2 id = q.id[lane];
+    const double2 mxy = q.xy[64 + lane];  // slot 127 is the dummy: never a live entry
+    const uint2 mid = q.id[64 + lane];
+    __builtin_amdgcn_wave_barrier();
     const uint32_t k = id.y & 0x7FFFFFFu;
-    const double lm = lm_of((uint64_t)tc.q * k, tc.l4_d);
-    const double ax2 = 2.0 * __builtin_fabs(xy.x), ay2 = 2.0 * __builtin_fabs(xy.y);
     const uint32_t terms = in ? id.y >> 28 : 0u;
-    bool pass = in & (terms == 0u);
-    for (uint32_t fl = terms; __builtin_amdgcn_ballot_w64(fl != 0u); fl &= fl - 1u) {
-        const uint32_t t = (uint32_t)__builtin_ctz(fl | 16u);  // 4 on lanes without a term left
-        const double a = t < 2u ? ax2 : ay2;
-        pass |= (fl != 0u) & root_near_integer((t & 1u) ? lm + a : lm - a);
-    }
+    const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without a term
+    const double lm = lm_of((uint64_t)tc.q * k, tc.l4_d);
+    const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
+    const bool hit = (terms != 0u) & root_near_integer((t & 1u) ? lm + a : lm - a);
+    const bool pass = in & ((terms == 0u) | hit);
     if (__builtin_amdgcn_ballot_w64(pass)) {
         const uint32_t ui = id.x & MAX_LEFT_ENTRIES, jj = id.x >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
         const uint32_t kflags = k | ((id.y >> 27) & 1u) << 30 | (terms == 0u ? 0x80000000u : 0u);
         push_global(P, pass, DefItem{tc.left, ui, vidx, kflags});
     }
+    const uint32_t rest = terms & (terms - 1u);
+    const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
+    q.xy[lane < keep ? lane : (uint32_t)(QCAP - 1)] = mxy;
+    q.id[lane < keep ? lane : (uint32_t)(QCAP - 1)] = mid;
+    const uint32_t slot = rest != 0u ? keep + lane_rank(again) : (uint32_t)(QCAP - 1);
+    q.xy[slot] = xy;
+    q.id[slot] = make_uint2(id.x, (id.y & 0x0FFFFFFFu) | rest << 28);
+    __builtin_amdgcn_wave_barrier();
+    q.n = keep + (uint32_t)__popcll(again);
 }
 
+// Append the lanes with pred set to the queue; full waves of entries are
+// drained (the queue never holds more than 127 live entries).
 __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCtx& tc, bool pred, double X, double Y,
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
@@ -466,29 +480,11 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
     q.xy[slot] = make_double2(X, Y);
     q.id[slot] = make_uint2(idlo, idhi);
     q.n += (uint32_t)__popcll(mask);
-    if (q.n >= 64) {
-        __builtin_amdgcn_wave_barrier();
-        root_drain(P, q, tc, 64);
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t lane = __lane_id();
-        const double2 mxy = q.xy[64 + lane];
-        const uint2 mid = q.id[64 + lane];
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t to = lane + 64 < q.n ? lane : (uint32_t)(QCAP - 1);
-        q.xy[to] = mxy;
-        q.id[to] = mid;
-        __builtin_amdgcn_wave_barrier();
-        q.n -= 64;
-    }
+    while (q.n >= 64) root_round(P, q, tc);
 }
 
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
-    if (q.n) {
-        __builtin_amdgcn_wave_barrier();
-        root_drain(P, q, tc, q.n);
-        q.n = 0;
-        __builtin_amdgcn_wave_barrier();
-    }
+    while (q.n) root_round(P, q, tc);
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
@@ -516,12 +512,9 @@ __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileC
     const bool ok = !und;
     const bool o2 = a2 & !eq & ok;
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
-    cnt += (uint32_t)f0;
-    cnt += (uint32_t)f1;
-    cnt += (uint32_t)f2;
-    cnt += (uint32_t)f3;
     const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
-    root_push(P, rq, tc, und | (terms != 0u), X, Y, idlo, kq | terms << 28);
+    cnt += (uint32_t)__builtin_popcount(terms);
+    root_push(P, rq, tc, (terms | (und ? 16u : 0u)) != 0u, X, Y, idlo, kq | terms << 28);
 }
 
 // The per-pair context and residues of one partner entry v (lane data).
@@ -675,27 +668,31 @@ __global__ __launch_bounds__(WG, 5) void cell_kernel(HSS_CELL_ARGS) {
 // Per chunk of 64 partner entries (one per lane, mapped as in cell_kernel):
 //   phase 1 (lane = partner entry v): the central u leaf as in cell_kernel;
 //     the index ranges of the left entries whose leaf u*v or u*conj(v) can
-//     count; the lane's partner data, fast-path context and ranges go to a
-//     per-wave LDS record;
-//   phase 2 (lane = one (v, range entry) item): the wave walks the flattened
-//     items of all 64 partners 64 at a time, each lane evaluating one leaf
-//     with leaf_fast from its owner's record and a vector load of u.
+//     count, cut into units of WIN_UNIT consecutive entries; the lane's
+//     partner data, fast-path context and ranges go to a per-wave LDS record;
+//   phase 2 (lane = one unit): the wave walks the flattened units of all 64
+//     partners 64 at a time; a lane reads its owner's record once and
+//     evaluates the unit's leaves with leaf_fast and vector loads of u.
 // Leaves outside the ranges count 0 exactly and are skipped.
 
+static constexpr uint32_t WIN_UNIT = 4;
+
 // 104 bytes: four blocks of four waves fit a CU's LDS with the root queues.
-// The narrowed range bounds are derived from the widened ones (n = w -+ tau).
+// The narrowed range bounds are derived from the widened ones (n = w -+ tau),
+// the area-test constants from the class of h (3, 9, 15, 45).
 struct WinRec {
     double c, d;  // partner v
     double h1_lo_w, h1_hi_w, h2_w, tau;
-    uint32_t inv24, thr22;
-    uint32_t cres;   // (c mod 180) << 8 | (d mod 180) << 24
-    uint32_t first;  // the partner's first item in the chunk's flattened items
-    uint32_t kj;
-    uint32_t jpk;     // partner entry << ID_U_BITS
+    uint32_t cres;    // c mod 180 | (d mod 180) << 8 | partner entry << 16
+    uint32_t kjh;     // k | class of h << 28
     uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
-    uint16_t cum[8];  // items before each range
+    uint16_t en[8];   // range ends
+    uint16_t ucum[8]; // units before each range
 };
 static_assert(sizeof(WinRec) == 104, "LDS budget of cell_win_kernel");
+
+static constexpr uint32_t AREA_INV[4] = {inv_mod_2_24(3), inv_mod_2_24(9), inv_mod_2_24(15), inv_mod_2_24(45)};
+static constexpr uint32_t AREA_THR[4] = {0x3FFFFFu / 3, 0x3FFFFFu / 9, 0x3FFFFFu / 15, 0x3FFFFFu / 45};
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
     __shared__ double2 s_qxy[WAVES][QCAP];
@@ -731,75 +728,81 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, v.ctx.tau, 2.0 * S.l2_d * m2, w);
             const double pv = psi_of_f(v.c, v.d);
             uint32_t rs[8], re[8];
-            uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
-            uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
-            n0 = act ? n0 : 0u;
-            n1 = (act && !v.vc) ? n1 : 0u;
-            const uint32_t cntv = n0 + n1;
-            evald += act ? 1u + cntv : 0u;  // the central leaf and the items
-            const uint32_t icl = wave_incl_scan(cntv);
-            const uint32_t excl = icl - cntv;  // first item of this lane's partner
-            const uint32_t items = __builtin_amdgcn_readlane(icl, 63);
+            const uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
+            const uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
+            const bool live0 = act && n0 != 0, live1 = act && !v.vc && n1 != 0;
             __builtin_amdgcn_wave_barrier();
             WinRec& R = rec[lane];
+            uint32_t items = 0, units = 0;
+            for (int p = 0; p < 8; ++p) {
+                const uint32_t len = (p < 4 ? live0 : live1) ? re[p] - rs[p] : 0u;
+                R.st[p] = (uint16_t)rs[p];
+                R.en[p] = (uint16_t)(rs[p] + len);
+                R.ucum[p] = (uint16_t)units;
+                items += len;
+                units += (len + WIN_UNIT - 1) / WIN_UNIT;
+            }
+            evald += act ? 1u + items : 0u;  // the central leaf and the items
+            const uint32_t inv = v.ctx.inv24;
+            const uint32_t hcl = inv == AREA_INV[0] ? 0u : inv == AREA_INV[1] ? 1u : inv == AREA_INV[2] ? 2u : 3u;
             R.c = v.c;
             R.d = v.d;
             R.h1_lo_w = v.ctx.h1_lo_w;
             R.h1_hi_w = v.ctx.h1_hi_w;
             R.h2_w = v.ctx.h2_w;
             R.tau = v.ctx.tau;
-            R.inv24 = v.ctx.inv24;
-            R.thr22 = v.ctx.thr22;
-            R.cres = v.c180s | v.d180s << 16;
-            R.first = excl;
-            R.kj = kj;
-            R.jpk = jj << ID_U_BITS;
-            uint32_t acc = 0;
-            for (int p = 0; p < 8; ++p) {
-                const bool live = p < 4 ? n0 != 0 : n1 != 0;
-                const uint32_t len = live ? re[p] - rs[p] : 0u;
-                R.st[p] = (uint16_t)rs[p];
-                R.cum[p] = (uint16_t)acc;
-                acc += len;
-            }
+            R.cres = (v.c180s >> 8) | (v.d180s & 0xFF00u) | jj << 16;
+            R.kjh = kj | hcl << 28;
+            const uint32_t ucl = wave_incl_scan(units);
+            const uint32_t uex = ucl - units;  // first unit of this lane's partner
+            const uint32_t nunits = __builtin_amdgcn_readlane(ucl, 63);
             __builtin_amdgcn_wave_barrier();
-            // ---- phase 2: lane = item
+            // ---- phase 2: lane = unit of WIN_UNIT consecutive entries of one range
             uint32_t carry = 0;
-            for (uint32_t i0 = 0; i0 < items; i0 += 64) {
-                const uint32_t it = i0 + (uint32_t)lane;
-                const uint32_t o = step_owner(mark, excl, cntv, i0, carry);
+            for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
+                const uint32_t wi = w0 + (uint32_t)lane;
+                const uint32_t o = step_owner(mark, uex, units, w0, carry);
+                const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
+                const bool liveu = wi < nunits;
+                const uint32_t wq = liveu ? wi - first : 0u;
                 const WinRec& Q = rec[o];
-                const bool live = it < items;
-                const uint32_t q = live ? it - Q.first : 0u;
                 uint32_t pc = 0;
-                for (int p = 1; p < 8; ++p) pc += Q.cum[p] <= q ? 1u : 0u;
-                const uint32_t ui0 = (uint32_t)Q.st[pc] + (q - (uint32_t)Q.cum[pc]);
-                const uint32_t ui = live && ui0 < nsort ? ui0 : 0u;  // in-bounds for every lane
+                for (int p = 1; p < 8; ++p) pc += Q.ucum[p] <= wq ? 1u : 0u;
+                const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
+                const uint32_t uend = liveu ? (uint32_t)Q.en[pc] : 0u;
                 const bool cj = pc >= 4;
-                const LeftFast u = lf[ui];
                 const double qc = Q.c, qd = Q.d;
                 const uint32_t qr = Q.cres;
-                const uint32_t qc180s = qr & 0xFFFFu, qd180s = qr >> 16;
+                const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
+                const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
                 LaneCtx qx;
                 qx.h1_lo_w = Q.h1_lo_w;
                 qx.h1_hi_w = Q.h1_hi_w;
                 qx.h2_w = Q.h2_w;
                 qx.tau = Q.tau;
-                qx.h1_lo_n = Q.h1_lo_w + Q.tau;
-                qx.h1_hi_n = Q.h1_hi_w - Q.tau;
-                qx.h2_n = Q.h2_w - Q.tau;
-                qx.inv24 = Q.inv24;
-                qx.thr22 = Q.thr22;
-                const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                const double t1 = u.im * qd, t2 = u.re * qd;
-                const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
-                const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
-                const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
-                const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
-                const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
-                const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
-                const uint32_t kq = Q.kj | (cj ? 1u << 27 : 0u);
-                leaf_fast(P, rq, tc, live, X, Y, vXs, vYs, qx, kq, ui | Q.jpk, cnt);
+                qx.h1_lo_n = qx.h1_lo_w + qx.tau;
+                qx.h1_hi_n = qx.h1_hi_w - qx.tau;
+                qx.h2_n = qx.h2_w - qx.tau;
+                qx.inv24 = qh == 0u ? AREA_INV[0] : qh == 1u ? AREA_INV[1] : qh == 2u ? AREA_INV[2] : AREA_INV[3];
+                qx.thr22 = qh == 0u ? AREA_THR[0] : qh == 1u ? AREA_THR[1] : qh == 2u ? AREA_THR[2] : AREA_THR[3];
+                const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
+                const uint32_t jpk = (qr >> 16) << ID_U_BITS;
+#pragma unroll
+                for (uint32_t s = 0; s < WIN_UNIT; ++s) {
+                    const uint32_t ui1 = ui0 + s;
+                    const bool live = ui1 < uend;
+                    const uint32_t ui = live ? ui1 : 0u;  // in-bounds for every lane (uend <= nsort)
+                    const LeftFast u = lf[ui];
+                    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                    const double t1 = u.im * qd, t2 = u.re * qd;
+                    const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
+                    const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
+                    const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
+                    const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
+                    const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
+                    const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
+                    leaf_fast(P, rq, tc, live, X, Y, vXs, vYs, qx, kq, ui | jpk, cnt);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         });

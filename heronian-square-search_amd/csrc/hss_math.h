@@ -185,7 +185,25 @@ HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, 
 
 // can an FP64 estimate zz (relative error < 2^-49, zz >= 1) be an exact fourth power?
 HSS_HD inline bool root_near_integer(double zz) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // zz^(1/4) from two hardware reciprocal square roots (zz^-1/2, then its
+    // own, each within 2^-22 relative even on a pessimistic reading of their
+    // 2^29-ulp bound), refined by Newton steps on r^4 = zz with the derivative
+    // from the first estimate (1/(4 r^3) ~ r t^2 / 4, t = zz^-1/2): each step
+    // multiplies the relative error by at most ~2^-21 (the derivative's error)
+    // plus rounding, so three steps leave < 2^-50 relative, as the two
+    // correctly rounded square roots would (|r - n| < 3e-8 for n <= 2^26).
+    const double t = __builtin_amdgcn_rsq(zz);
+    double r = __builtin_amdgcn_rsq(t);
+    const double w = 0.25 * (t * t);
+    for (int i = 0; i < 3; ++i) {
+        const double r2 = r * r;
+        const double d = __builtin_fma(r2, r2, -zz);  // r^4 - zz
+        r = __builtin_fma(-d, r * w, r);
+    }
+#else
     double r = __builtin_sqrt(__builtin_sqrt(zz));
+#endif
     double f = r - __builtin_rint(r);
     return (f < 0 ? -f : f) <= ROOT_EPS;
 }

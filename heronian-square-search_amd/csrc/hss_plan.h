@@ -37,11 +37,10 @@ inline int weight_class(uint32_t H) {
 }
 
 // Defaults: hss_autotune on MI355X, calibration roots drawn like the uneven
-// LIMIT 1e7 workload with another seed (tiles_per_wave 32 measured 1% faster
-// than 16 but plans 35% longer on the host).
+// LIMIT 1e7 workload with another seed.
 struct Tuning {
-    uint32_t tiles_per_wave = 16;  // tile size: est. work / (waves * tiles_per_wave)
-    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.3, 0.2, 0.5, 0.6, 0.5, 0.5, 0.5};
+    uint32_t tiles_per_wave = 32;  // tile size: est. work / (waves * tiles_per_wave)
+    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.3, 0.3, 0.5, 0.6, 0.6, 0.3, 0.5};
     double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf (tile order)
     double win_partner_cost = 4;  // per partner entry of a window tile, in plain-leaf units
 };
