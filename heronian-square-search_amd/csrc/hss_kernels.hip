@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
 
 static constexpr int WG = 256;
 static constexpr int WAVES = WG / 64;
-static constexpr int QCAP = 128;
+static constexpr int QCAP = 128;  // root queue ring (a power of two), plus one dummy slot
 
 // Inclusive prefix sum / maximum over the wavefront with DPP: shifts by 1, 2,
 // 4, 8 within each row of 16 lanes, then the last lane of rows 0 and 2 into
@@ -407,18 +407,19 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
     if (pred && slot < P.def_cap) P.def[slot] = it;
 }
 
-// Per-wave LDS queue of leaves for the deferred buffer: leaves with counted
+// Per-wave LDS ring of leaves for the deferred buffer: leaves with counted
 // terms awaiting the root-proximity test, and leaves the FP64 band left
 // undecided.  Entry: X, Y of z/2 and the packed id: (u | partner entry << 17,
 // k | conj << 27 | counted terms << 28), the terms as bits 0-3 for lm - 2|X|,
 // lm + 2|X|, lm - 2|Y|, lm + 2|Y|; an entry without counted terms is undecided.
+// Slots head .. head + n - 1 (mod QCAP) are live; slot QCAP takes the writes
+// of lanes without an entry.
 struct RootQueue {
     double2* xy;
     uint2* id;
-    uint32_t n;
+    uint32_t head, n;
 };
 
-// What a drain needs besides the queue: the tile's left set and position.
 struct TileCtx {
     uint32_t left;
     uint32_t q;      // m = q * k
@@ -427,27 +428,26 @@ struct TileCtx {
 
 __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj << ID_U_BITS; }
 
-// One round over the first min(n, 64) queued leaves, one per lane: an
+// One round over the oldest min(n, 64) queued leaves, one per lane: an
 // undecided leaf goes to the deferred buffer for the full exact count; a leaf
 // with counted terms has its lowest remaining term root-tested (a survivor goes
 // to the deferred buffer) and is queued again while terms remain, so every
-// round tests a full wave of terms.  The entries after the first 64 move to
-// the front, the re-queued ones follow them.
+// round tests a full wave of terms.
 __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
     const uint32_t count = q.n < 64u ? q.n : 64u;
-    const uint32_t keep = q.n - count;
     const bool in = lane < count;
     __builtin_amdgcn_wave_barrier();
-    const double2 xy = q.xy[lane];
-    const uint2 id = q.id[lane];
-    const double2 mxy = q.xy[64 + lane];  // slot 127 is the dummy: never a live entry
-    const uint2 mid = q.id[64 + lane];
+    const uint32_t at = (q.head + lane) & (uint32_t)(QCAP - 1);
+    const double2 xy = q.xy[at];
+    const uint2 id = q.id[at];
     __builtin_amdgcn_wave_barrier();
+    q.head = (q.head + count) & (uint32_t)(QCAP - 1);
+    q.n -= count;
     const uint32_t k = id.y & 0x7FFFFFFu;
     const uint32_t terms = in ? id.y >> 28 : 0u;
     const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without a term
-    const double lm = lm_of((uint64_t)tc.q * k, tc.l4_d);
+    const double lm = lm_of(tc.q, k, tc.l4_d);
     const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
     const bool hit = (terms != 0u) & root_near_integer((t & 1u) ? lm + a : lm - a);
     const bool pass = in & ((terms == 0u) | hit);
@@ -459,13 +459,13 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     }
     const uint32_t rest = terms & (terms - 1u);
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
-    q.xy[lane < keep ? lane : (uint32_t)(QCAP - 1)] = mxy;
-    q.id[lane < keep ? lane : (uint32_t)(QCAP - 1)] = mid;
-    const uint32_t slot = rest != 0u ? keep + lane_rank(again) : (uint32_t)(QCAP - 1);
-    q.xy[slot] = xy;
-    q.id[slot] = make_uint2(id.x, (id.y & 0x0FFFFFFFu) | rest << 28);
-    __builtin_amdgcn_wave_barrier();
-    q.n = keep + (uint32_t)__popcll(again);
+    if (again) {
+        const uint32_t slot = rest != 0u ? (q.head + q.n + lane_rank(again)) & (uint32_t)(QCAP - 1) : (uint32_t)QCAP;
+        q.xy[slot] = xy;
+        q.id[slot] = make_uint2(id.x, (id.y & 0x0FFFFFFFu) | rest << 28);
+        __builtin_amdgcn_wave_barrier();
+        q.n += (uint32_t)__popcll(again);
+    }
 }
 
 // Append the lanes with pred set to the queue; full waves of entries are
@@ -474,9 +474,7 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
-    // lanes without an item write the dummy slot QCAP-1 (never a live entry:
-    // the queue holds at most 63 + 64 = 127 items), so the store is unconditional
-    const uint32_t slot = pred ? q.n + lane_rank(mask) : (uint32_t)(QCAP - 1);
+    const uint32_t slot = pred ? (q.head + q.n + lane_rank(mask)) & (uint32_t)(QCAP - 1) : (uint32_t)QCAP;
     q.xy[slot] = make_double2(X, Y);
     q.id[slot] = make_uint2(idlo, idhi);
     q.n += (uint32_t)__popcll(mask);
@@ -620,12 +618,12 @@ __device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t
         const LeftDesc* __restrict__ lefts, const PosDesc* __restrict__ pos,                            \
         const LeftFast* __restrict__ lfast
 
-__global__ __launch_bounds__(WG, 5) void cell_kernel(HSS_CELL_ARGS) {
-    __shared__ double2 s_qxy[WAVES][QCAP];
-    __shared__ uint2 s_qid[WAVES][QCAP];
+__global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
+    __shared__ double2 s_qxy[WAVES][QCAP + 1];
+    __shared__ uint2 s_qid[WAVES][QCAP + 1];
     __shared__ uint32_t s_mark[WAVES][65];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_qxy[wave], s_qid[wave], 0};
+    RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
     uint32_t* mark = s_mark[wave];
     for (;;) {
         // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
@@ -695,12 +693,12 @@ static constexpr uint32_t AREA_INV[4] = {inv_mod_2_24(3), inv_mod_2_24(9), inv_m
 static constexpr uint32_t AREA_THR[4] = {0x3FFFFFu / 3, 0x3FFFFFu / 9, 0x3FFFFFu / 15, 0x3FFFFFu / 45};
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
-    __shared__ double2 s_qxy[WAVES][QCAP];
-    __shared__ uint2 s_qid[WAVES][QCAP];
+    __shared__ double2 s_qxy[WAVES][QCAP + 1];
+    __shared__ uint2 s_qid[WAVES][QCAP + 1];
     __shared__ WinRec s_rec[WAVES][64];
     __shared__ uint32_t s_mark[WAVES][65];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_qxy[wave], s_qid[wave], 0};
+    RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
     WinRec* rec = s_rec[wave];
     uint32_t* mark = s_mark[wave];
     for (;;) {

@@ -125,6 +125,13 @@ HSS_HD inline double lm_of(uint64_t m, double l4_d) {
     return l4_d + m2 * m2;
 }
 
+// the same for m = q * k given as factors (their FP64 product is exact: m < 2^26)
+HSS_HD inline double lm_of(uint32_t q, uint32_t k, double l4_d) {
+    const double md = (double)q * (double)k;
+    const double m2 = md * md;
+    return l4_d + m2 * m2;
+}
+
 // Per-pair context.  l4_d, n4_d: FP64 values of l^4 and N^4; l2_d = l^2;
 // l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5.
 HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
@@ -188,24 +195,30 @@ HSS_HD inline bool root_near_integer(double zz) {
 #if defined(__HIP_DEVICE_COMPILE__)
     // zz^(1/4) from two hardware reciprocal square roots (zz^-1/2, then its
     // own, each within 2^-22 relative even on a pessimistic reading of their
-    // 2^29-ulp bound), refined by Newton steps on r^4 = zz with the derivative
-    // from the first estimate (1/(4 r^3) ~ r t^2 / 4, t = zz^-1/2): each step
-    // multiplies the relative error by at most ~2^-21 (the derivative's error)
-    // plus rounding, so three steps leave < 2^-50 relative, as the two
-    // correctly rounded square roots would (|r - n| < 3e-8 for n <= 2^26).
+    // 2^29-ulp bound), refined by two Newton steps on r^4 = zz with the
+    // derivative from the first estimate (1/(4 r^3) ~ r t^2 / 4, t = zz^-1/2):
+    // a step takes relative error e to ~2.5 e^2 + 2^-21 e, so r is within
+    // 2^-38 of zz^(1/4) (0.001 absolute for n <= 2^26) and rint(r) is the only
+    // candidate n.  Then n^4 - zz (n^2 exact, one rounding in the fma) must be
+    // within 2^-47 zz: a true fourth power Z = n^4 has |zz - Z| < 2^-49.3 Z,
+    // and a random zz passes with probability ~2^-48 n.
     const double t = __builtin_amdgcn_rsq(zz);
     double r = __builtin_amdgcn_rsq(t);
     const double w = 0.25 * (t * t);
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 2; ++i) {
         const double r2 = r * r;
         const double d = __builtin_fma(r2, r2, -zz);  // r^4 - zz
         r = __builtin_fma(-d, r * w, r);
     }
+    const double n = __builtin_rint(r);
+    const double n2 = n * n;
+    const double e = __builtin_fma(n2, n2, -zz);
+    return __builtin_fabs(e) <= zz * 0x1p-47;
 #else
     double r = __builtin_sqrt(__builtin_sqrt(zz));
-#endif
     double f = r - __builtin_rint(r);
     return (f < 0 ? -f : f) <= ROOT_EPS;
+#endif
 }
 
 // One leaf z/2 = X + iY with area residues vX = X, vY = Y (mod 180).  cnt is
