@@ -149,6 +149,47 @@ double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>
     return est;
 }
 
+bool split_by_work(uint64_t limit, uint64_t first, uint64_t last, uint32_t parts,
+                   const std::vector<uint32_t>& small_primes, const uint32_t* off, uint64_t n_max, uint64_t* bounds,
+                   std::string& err) {
+    if (!parts || !bounds) {
+        err = "need at least one part";
+        return false;
+    }
+    if (limit < 1 || limit > n_max || first < 1 || first > last || last > limit) {
+        err = "interval outside [1, limit] or limit outside [1, n_max]";
+        return false;
+    }
+    const uint64_t n = last - first + 1;
+    std::vector<double> work(n);
+    std::atomic<bool> bad{false};
+    const uint32_t chunks = (uint32_t)std::min<uint64_t>(256, (n + 255) / 256);
+    parallel_chunks((uint32_t)n, chunks, [&](uint32_t b, uint32_t e, uint32_t) {
+        for (uint32_t i = b; i < e; ++i) {
+            work[i] = estimate_position(first + i, limit, small_primes, off, n_max);
+            if (work[i] < 0) bad = true;
+        }
+    });
+    if (bad) {
+        err = "position cannot be planned";
+        return false;
+    }
+    double total = 0;
+    for (double w : work) total += w;
+    bounds[0] = first;
+    double acc = 0;
+    uint64_t l = first;
+    for (uint32_t i = 1; i < parts; ++i) {
+        const double target = total * i / parts;
+        while (l <= last && acc + work[l - first] < target) acc += work[l++ - first];
+        // l is the first position whose inclusive prefix reaches the target
+        bounds[i] = l <= last ? l + 1 : last + 1;
+        if (bounds[i] < bounds[i - 1]) bounds[i] = bounds[i - 1];
+    }
+    bounds[parts] = last + 1;
+    return true;
+}
+
 double tile_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1, int samples) {
     const double l2 = (double)(l * l), l4 = l2 * l2;
     const double nd = (double)(limit * limit), n4 = nd * nd;

@@ -40,7 +40,7 @@ inline int weight_class(uint32_t H) {
 // LIMIT 1e7 workload with another seed.
 struct Tuning {
     uint32_t tiles_per_wave = 32;  // tile size: est. work / (waves * tiles_per_wave)
-    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.45, 0.3, 0.5, 0.6, 1.0, 0.3, 0.5};
+    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.45, 1.0, 0.5, 0.6, 1.0, 0.8, 0.5};
     double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf (tile order)
     double win_partner_cost = 4;  // per partner entry of a window tile, in plain-leaf units
 };
@@ -93,5 +93,15 @@ uint32_t gaussian_weight(uint64_t l, const std::vector<uint32_t>& primes);
 // planned (l outside [1, limit], limit > n_max, too many distinct primes).
 double estimate_position(uint64_t l, uint64_t limit, const std::vector<uint32_t>& small_primes, const uint32_t* off,
                          uint64_t n_max);
+
+// Contiguous split of the root positions [first, last] into `parts` intervals
+// of about equal estimated work: part i is [bounds[i], bounds[i+1] - 1],
+// bounds[0] = first, bounds[parts] = last + 1, and bounds[i] is one past the
+// first position whose inclusive prefix of work reaches i/parts of the total
+// (so a part is empty only when a single position outweighs a share).  False with
+// err set when an input is out of range or a position cannot be planned.
+bool split_by_work(uint64_t limit, uint64_t first, uint64_t last, uint32_t parts,
+                   const std::vector<uint32_t>& small_primes, const uint32_t* off, uint64_t n_max, uint64_t* bounds,
+                   std::string& err);
 
 }  // namespace hss

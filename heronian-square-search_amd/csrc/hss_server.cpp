@@ -517,6 +517,16 @@ extern "C" void hss_server_destroy(hss_server* S) {
     delete S;
 }
 
+extern "C" int hss_split_by_work(const hss_ctx* c, uint64_t limit, uint64_t first, uint64_t last, uint32_t parts,
+                                 uint64_t* bounds) {
+    if (!c || !bounds) return fail(HSS_EINVAL, "null argument");
+    if (!parts || parts > (1u << 20)) return fail(HSS_EINVAL, "parts outside [1, 2^20]");
+    std::string err;
+    if (!split_by_work(limit, first, last, parts, c->small_primes, c->h_off.data(), c->nmax, bounds, err))
+        return fail(HSS_EINVAL, err);
+    return HSS_OK;
+}
+
 extern "C" int hss_estimate(const hss_ctx* c, uint64_t limit, uint64_t first, uint64_t last, double* cells) {
     if (!c || !cells) return fail(HSS_EINVAL, "null argument");
     if (limit < 1 || limit > c->nmax) return fail(HSS_EINVAL, "limit outside [1, n_max]");

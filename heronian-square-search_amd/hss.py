@@ -126,6 +126,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
                                  ctypes.POINTER(ctypes.c_double)]
     lib.hss_verify_triple.argtypes = [u64, u64, u64, u64, u64]
     lib.hss_estimate.argtypes = [vp, u64, u64, u64, ctypes.POINTER(ctypes.c_double)]
+    lib.hss_split_by_work.argtypes = [vp, u64, u64, u64, u32, ctypes.POINTER(u64)]
     lib.hss_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     lib.hss_server_submit.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64)]
     lib.hss_server_wait.argtypes = [vp, u64, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
@@ -267,6 +268,13 @@ class Context:
         out = ctypes.c_double()
         _check(_lib.hss_estimate(self._h, limit, first, last, ctypes.byref(out)))
         return out.value
+
+    def split_by_work(self, limit: int, first: int, last: int, parts: int) -> list[int]:
+        """Bounds of `parts` contiguous intervals of [first, last] with about
+        equal estimated work: interval i is [b[i], b[i+1] - 1]."""
+        out = (ctypes.c_uint64 * (parts + 1))()
+        _check(_lib.hss_split_by_work(self._h, limit, first, last, parts, out))
+        return list(out)
 
 
 @dataclass

@@ -185,6 +185,19 @@ uint32_t hss_gaussian_weight(uint64_t l);
  */
 int hss_estimate(const hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t last, double* cells);
 
+/*
+ * Split the root positions [first, last] at `limit` into `parts` contiguous
+ * intervals of about equal estimated work (hss_estimate's cells): interval i
+ * is [bounds[i], bounds[i+1] - 1], bounds[0] = first, bounds[parts] = last + 1;
+ * bounds[i] is one past the first position whose inclusive prefix of work
+ * reaches i/parts of the total.  `bounds` holds parts + 1 values.  Replaces
+ * the reference supervisor's fixed 12500-position interval per GPU
+ * (scripts/gpu_queue.py:224-226) and its idle-GPU rebalancing
+ * (scripts/gpu_rebalance.py:65-108) with a split by work.  Host arithmetic.
+ */
+int hss_split_by_work(const hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t last, uint32_t parts,
+                      uint64_t* bounds);
+
 /* ------------------------------------------------------------ served form
  *
  * A server drives one context from its own thread.  Clients on any thread

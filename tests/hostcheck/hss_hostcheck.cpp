@@ -339,6 +339,14 @@ double hc_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, u
     return tile_window_fraction(l, limit, q, k0, k1, 33);
 }
 
+// The planner's split of [first, last] into parts of about equal work
+// (bounds: parts + 1 values); 0 on success.
+int hc_split_by_work(uint64_t nmax, uint64_t limit, uint64_t first, uint64_t last, uint32_t parts, uint64_t* bounds) {
+    if (g_tables.n != nmax) build(g_tables, nmax);
+    std::string err;
+    return split_by_work(limit, first, last, parts, g_tables.small, g_tables.off.data(), nmax, bounds, err) ? 0 : 1;
+}
+
 // Planner work estimate of one position (cells), or -1.
 double hc_estimate(uint64_t nmax, uint64_t l, uint64_t limit) {
     if (g_tables.n != nmax) build(g_tables, nmax);

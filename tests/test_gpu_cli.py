@@ -313,3 +313,21 @@ def test_gpu_serve_checkpoints(tmp_path):
     with hss.Context(limit) as ctx:
         total = ctx.search_interval(limit, first, last)[0]
     assert stopped["candidates"] + final["candidates"] == total
+
+
+def test_queue_on_gpu(tmp_path):
+    """The interval queue (hss_queue.py, counterpart of scripts/gpu_queue.py)
+    on this box's GPU: the split by work is computed on GPU 0, one
+    gpu_search_v42 process runs the part, and its files are the reference's
+    (STATE complete, +512 PROGRESS chain in the log, the full5000 result)."""
+    import sys
+    c = CASES["full5000"]
+    p = subprocess.run([sys.executable, str(PKG / "hss_queue.py"), str(c["limit"]), str(c["first"]), str(c["last"]),
+                        "--gpus", "0", "--suffix", "q", "--dir", str(tmp_path)],
+                       capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, (p.stdout, p.stderr)
+    assert "PART gpu=0 range=1-5000" in p.stdout and "DONE gpu=0 range=1-5000" in p.stdout
+    st = read_state(tmp_path / "gpu-0q-state.txt")
+    assert (st["limit"], st["end"], st["resume"], st["candidates"]) == (5000, 5000, 5001, c["candidates"])
+    assert chain_end((tmp_path / "gpu-0q.log").read_text(), 1, 5000) == 5001
+    assert (tmp_path / "gpu-0q-results.txt").read_text() == c["results"]

@@ -158,6 +158,47 @@ def test_estimate_bounds_leaves(hc):
     assert hc.hc_estimate(ctypes.c_uint64(70000), ctypes.c_uint64(5001), ctypes.c_uint64(5000)) == -1.0
 
 
+def split(hc, nmax, limit, first, last, parts):
+    out = (ctypes.c_uint64 * (parts + 1))()
+    rc = hc.hc_split_by_work(ctypes.c_uint64(nmax), ctypes.c_uint64(limit), ctypes.c_uint64(first),
+                             ctypes.c_uint64(last), ctypes.c_uint32(parts), out)
+    return rc, list(out)
+
+
+@pytest.mark.parametrize("limit,first,last,parts", [(70000, 1, 3000, 8), (70000, 60000, 70000, 5),
+                                                    (5000, 1, 5000, 3), (70000, 11789, 11789, 4),
+                                                    (70000, 100, 131, 1)])
+def test_split_by_work(hc, limit, first, last, parts):
+    """Interval split by the planner's work estimate (the sharding of an
+    interval over GPUs): contiguous cover of [first, last]; every boundary is
+    one past the position whose inclusive work prefix first reaches its share;
+    no part exceeds its share by more than one position's work."""
+    est = hc.hc_estimate
+    est.restype = ctypes.c_double
+    work = [est(ctypes.c_uint64(70000), ctypes.c_uint64(l), ctypes.c_uint64(limit)) for l in range(first, last + 1)]
+    assert min(work) >= 0
+    rc, b = split(hc, 70000, limit, first, last, parts)
+    assert rc == 0
+    assert b[0] == first and b[-1] == last + 1 and b == sorted(b)
+    total = sum(work)
+    prefix = [0.0]
+    for w in work:
+        prefix.append(prefix[-1] + w)  # prefix[i] = work of first .. first + i - 1
+    for i in range(1, parts):
+        x = b[i] - 1 - first  # index of the crossing position
+        target = total * i / parts
+        assert prefix[x + 1] >= target * (1 - 1e-12) and (x == 0 or prefix[x] < target * (1 + 1e-12)), (i, b)
+    for i in range(parts):
+        part = prefix[b[i + 1] - first] - prefix[b[i] - first]
+        assert part <= total / parts + max(work) * (1 + 1e-12)
+
+
+def test_split_by_work_rejects(hc):
+    assert split(hc, 70000, 70000, 10, 9, 2)[0] != 0
+    assert split(hc, 70000, 70000, 1, 10, 0)[0] != 0
+    assert split(hc, 70000, 70001, 1, 10, 2)[0] != 0
+
+
 @pytest.mark.parametrize("limit", [5000, 70000])
 def test_window_mode_per_position_golden(hc, golden, window_everywhere, limit):
     """Every sortable left set in window mode: the per-position counts still
