@@ -487,12 +487,11 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
 // for lane data, with area residues pre-shifted (vXs = vX << 8).  Adds the
-// decided count to cnt and queues the leaf when it has counted terms (for the
-// root-proximity test) or is undecided (for the exact count).  kq = k | conj
-// << 27; idlo = pack_ui(ui, partner entry).
-__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
-                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
-                                 uint32_t& cnt) {
+// decided count to cnt and returns the leaf's queue flags: its counted terms
+// (bits 0-3) and 16 when the FP64 band left it undecided; 0 when there is
+// nothing to queue.
+__device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t vXs, uint32_t vYs, const LaneCtx& c,
+                                       uint32_t& cnt) {
     // bitwise logic on lane masks: no short-circuit branches
     const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
     const bool a1 = live & area_ok_s(vYs, c);
@@ -512,7 +511,21 @@ __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileC
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
     const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
     cnt += (uint32_t)__builtin_popcount(terms);
-    root_push(P, rq, tc, (terms | (und ? 16u : 0u)) != 0u, X, Y, idlo, kq | terms << 28);
+    return terms | (und ? 16u : 0u);
+}
+
+// Queue a decided leaf (flags from leaf_decide; kq = k | conj << 27, idlo =
+// pack_ui(ui, partner entry)): with counted terms for the root-proximity test,
+// undecided for the exact count.
+__device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t flags, double X,
+                                  double Y, uint32_t kq, uint32_t idlo) {
+    root_push(P, rq, tc, flags != 0u, X, Y, idlo, kq | (flags & 15u) << 28);
+}
+
+__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
+                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
+                                 uint32_t& cnt) {
+    leaf_queue(P, rq, tc, leaf_decide(live, X, Y, vXs, vYs, c, cnt), X, Y, kq, idlo);
 }
 
 // The per-pair context and residues of one partner entry v (lane data).
@@ -648,10 +661,12 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 const double t1 = u.im * v.d, t2 = u.re * v.d;
                 const uint32_t pac = __umul24(ua, v.c180s), pbd = __umul24(ub, v.d180s);
                 const uint32_t pad = __umul24(ua, v.d180s), pbc = __umul24(ub, v.c180s);
-                leaf_fast(P, rq, tc, act, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2),
-                          pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, kj, ui | jpk, cnt);
-                leaf_fast(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, t1), __builtin_fma(u.im, v.c, -t2),
-                          pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, kj | 1u << 27, ui | jpk, cnt);
+                const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
+                const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
+                const uint32_t f0 = leaf_decide(act, X0, Y0, pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, cnt);
+                const uint32_t f1 = leaf_decide(act & !v.vc, X1, Y1, pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, cnt);
+                leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
+                leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
             }
             central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
         });
@@ -785,6 +800,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 qx.thr22 = qh == 0u ? AREA_THR[0] : qh == 1u ? AREA_THR[1] : qh == 2u ? AREA_THR[2] : AREA_THR[3];
                 const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
                 const uint32_t jpk = (qr >> 16) << ID_U_BITS;
+                double UX[WIN_UNIT], UY[WIN_UNIT];
+                uint32_t UF[WIN_UNIT], UI[WIN_UNIT];
 #pragma unroll
                 for (uint32_t s = 0; s < WIN_UNIT; ++s) {
                     const uint32_t ui1 = ui0 + s;
@@ -795,12 +812,15 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const double t1 = u.im * qd, t2 = u.re * qd;
                     const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
                     const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
-                    const double X = __builtin_fma(u.re, qc, cj ? t1 : -t1);
-                    const double Y = __builtin_fma(u.im, qc, cj ? -t2 : t2);
+                    UX[s] = __builtin_fma(u.re, qc, cj ? t1 : -t1);
+                    UY[s] = __builtin_fma(u.im, qc, cj ? -t2 : t2);
                     const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
                     const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
-                    leaf_fast(P, rq, tc, live, X, Y, vXs, vYs, qx, kq, ui | jpk, cnt);
+                    UF[s] = leaf_decide(live, UX[s], UY[s], vXs, vYs, qx, cnt);
+                    UI[s] = ui;
                 }
+#pragma unroll
+                for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, UI[s] | jpk);
             }
             __builtin_amdgcn_wave_barrier();
         });
