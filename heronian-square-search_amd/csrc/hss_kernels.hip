@@ -537,8 +537,35 @@ struct Partner {
     LaneCtx ctx;
 };
 
-__device__ inline void load_partner(const CellParams& P, const LeftDesc& L, const PosDesc& S, bool act, uint32_t kj,
-                                    uint32_t jj, uint32_t Hj, uint32_t kdef, Partner& v) {
+// The pair context of m = q * k (make_lane_ctx).
+__device__ inline void pair_ctx(const LeftDesc& L, const PosDesc& S, uint32_t k, LaneCtx& c) {
+    const uint32_t m3 = (L.q3 * (k % 3u)) % 3u, m5 = (L.q5 * (k % 5u)) % 5u;
+    make_lane_ctx((uint64_t)L.q * k, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, c);
+}
+
+// Per-k record of the widened bounds (the narrowed ones are w -+ tau) and the
+// area-test constants: one per lane of a partner chunk, shared by the k's
+// half-set entries.
+struct KCtx {
+    double h1_lo_w, h1_hi_w, h2_w, tau;
+    uint32_t inv24, thr22;
+};
+
+__device__ inline void ctx_from_k(const KCtx& r, LaneCtx& c) {
+    c.h1_lo_w = r.h1_lo_w;
+    c.h1_hi_w = r.h1_hi_w;
+    c.h2_w = r.h2_w;
+    c.tau = r.tau;
+    c.h1_lo_n = r.h1_lo_w + r.tau;
+    c.h1_hi_n = r.h1_hi_w - r.tau;
+    c.h2_n = r.h2_w - r.tau;
+    c.inv24 = r.inv24;
+    c.thr22 = r.thr22;
+    c.lm = 0.0;  // unused by the fast path (the queue recomputes it)
+}
+
+__device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, uint32_t jj, uint32_t Hj,
+                                    uint32_t kdef, Partner& v) {
     const uint32_t kjq = act ? kj : kdef;  // in-bounds index for every lane (act: kj <= k1)
     v.kj = kj;
     v.jj = jj;
@@ -549,9 +576,6 @@ __device__ inline void load_partner(const CellParams& P, const LeftDesc& L, cons
     const uint32_t vr = P.cres[v.vidx];
     v.c180s = (vr & 0xFFFFu) << 8;
     v.d180s = (vr >> 8) & 0xFFFF00u;
-    const uint64_t m = (uint64_t)L.q * kj;
-    const uint32_t m3 = (L.q3 * (kj % 3u)) % 3u, m5 = (L.q5 * (kj % 5u)) % 5u;
-    make_lane_ctx(m, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, v.ctx);
 }
 
 // The central (real) entry u of the left set against v: leaf u*v only; with a
@@ -572,11 +596,13 @@ __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const Ti
 }
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
-// (k = kb + 2 lane) are filtered and their half-set entries flattened; f is
-// called for every 64 flattened entries with the lane's entry (act, kj, jj, Hj).
-template <typename F>
+// (k = kb + 2 lane) are filtered (on_chunk(k) is called with each lane's k
+// first) and their half-set entries flattened; f is called for every 64
+// flattened entries with the lane's entry (act, kj, jj, Hj) and the lane j of
+// its k.
+template <typename C, typename F>
 __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T, const LeftDesc& L, const PosDesc& S,
-                                          uint32_t* mark, uint64_t& pairs, F&& f) {
+                                          uint32_t* mark, uint64_t& pairs, C&& on_chunk, F&& f) {
     const uint32_t lane = __lane_id();
     for (uint32_t par = 0; par < 2; ++par) {
         const uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
@@ -589,6 +615,7 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
             uint32_t Hk = P.off[kq + 1] - P.off[kq];
             Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
             pairs += Hk ? 1 : 0;
+            on_chunk(k);
             const uint32_t incl = wave_incl_scan(Hk);
             const uint32_t excl = incl - Hk;
             const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
@@ -602,7 +629,7 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
                 // an entry beyond the owner's run never happens; the test keeps
                 // every index in bounds regardless
                 const bool act = e < total && jj < Hj;
-                f(act, kb + 2u * j, act ? jj : 0u, Hj);
+                f(act, kb + 2u * j, act ? jj : 0u, Hj, j);
             }
         }
     }
@@ -635,9 +662,11 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
     __shared__ double2 s_qxy[WAVES][QCAP + 1];
     __shared__ uint2 s_qid[WAVES][QCAP + 1];
     __shared__ uint32_t s_mark[WAVES][65];
+    __shared__ KCtx s_kctx[WAVES][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
     uint32_t* mark = s_mark[wave];
+    KCtx* kctx = s_kctx[wave];
     for (;;) {
         // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_tile, lane == 0 ? 1u : 0u));
@@ -649,9 +678,19 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const LeftFast* lf = lfast + L.off;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0;
-        for_partner_chunks(P, T, L, S, mark, pairs, [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj) {
+        // the pair context once per k (a k's half-set entries share it)
+        const auto on_chunk = [&](uint32_t k) {
+            LaneCtx c;
+            pair_ctx(L, S, k, c);
+            __builtin_amdgcn_wave_barrier();
+            kctx[lane] = KCtx{c.h1_lo_w, c.h1_hi_w, c.h2_w, c.tau, c.inv24, c.thr22};
+            __builtin_amdgcn_wave_barrier();
+        };
+        for_partner_chunks(P, T, L, S, mark, pairs, on_chunk,
+                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j) {
             Partner v;
-            load_partner(P, L, S, act, kj, jj, Hj, T.k0, v);
+            load_partner(P, act, kj, jj, Hj, T.k0, v);
+            ctx_from_k(kctx[j], v.ctx);
             leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
@@ -728,10 +767,12 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const uint32_t nsort = L.H - 1;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        for_partner_chunks(P, T, L, S, mark, pairs, [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj) {
+        for_partner_chunks(P, T, L, S, mark, pairs, [](uint32_t) {},
+                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t) {
             // ---- phase 1: lane = partner entry
             Partner v;
-            load_partner(P, L, S, act, kj, jj, Hj, T.k0, v);
+            load_partner(P, act, kj, jj, Hj, T.k0, v);
+            pair_ctx(L, S, kj, v.ctx);
             leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
             central_leaf(P, rq, tc, act, lf[nsort], nsort, v, cnt);
             // ranges of non-central entries
