@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
                                                    const LeftBuild* builds, const LeftDesc* lefts,
                                                    const uint32_t* vbase, const i128* vre, const i128* vim,
                                                    uint32_t* perm, uint32_t* btab) {
-    __shared__ double key[WIN_MAX_SORT];
+    __shared__ float key[WIN_MAX_SORT];  // psi_u rounded to FP32 (the window's error budget allows it)
     __shared__ uint16_t idx[WIN_MAX_SORT];
     const uint32_t nsorted = hdr->nsorted;
     for (uint32_t bi = blockIdx.x; bi < nsorted; bi += gridDim.x) {  // uniform per block
@@ -279,10 +279,10 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
         uint32_t n2 = 1;
         while (n2 < n) n2 <<= 1;
         for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) {
-            double k = 1e300;
+            float k = 3e38f;
             if (t < n) {
                 GInt z = left_value(b, t, vbase, vre, vim);
-                k = psi_of(i128_to_double(z.re), i128_to_double(z.im));
+                k = (float)psi_of(i128_to_double(z.re), i128_to_double(z.im));
             }
             key[t] = k;
             idx[t] = (uint16_t)t;
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
                     const uint32_t i = 2 * t - (t & (stride - 1));
                     const uint32_t j = i + stride;
                     const bool up = (i & size) == 0;
-                    const double ki = key[i], kj = key[j];
+                    const float ki = key[i], kj = key[j];
                     const bool sw = up ? (ki > kj || (ki == kj && idx[i] > idx[j]))
                                        : (ki < kj || (ki == kj && idx[i] < idx[j]));
                     if (sw) {

@@ -290,9 +290,10 @@ HSS_HD inline void eval_real_leaf(uint32_t vX, const LaneCtx& c, LeafEval& r) {
 // rounded arguments still enclose [s0, s1] and asinf is evaluated on an
 // enclosing interval; asinf (<= 4 ulp, < 5e-7 rad) and the partner angle from
 // atan2f on FP32 coordinates (coordinate rounding < 1.2e-7 rad, atan2f <= 3 ulp
-// < 3.6e-7 rad) add < 1e-6 rad together; the sort keys psi_u are FP64.  The
-// angle window is widened by WIN_EPS = 2e-6 rad.  FP32 keeps the register
-// footprint of the hot kernel low; the bounds only have to enclose.
+// < 3.6e-7 rad) add < 1e-6 rad together; the sort keys psi_u are computed in
+// FP64 and stored rounded to FP32 (< 1e-7 rad, psi < pi/2).  The angle window
+// is widened by WIN_EPS = 2e-6 rad.  FP32 keeps the register footprint of the
+// hot kernel and the sort's LDS low; the bounds only have to enclose.
 
 static constexpr double HALF_PI = 1.5707963267948966;
 static constexpr double WIN_EPS = 2e-6;

@@ -164,8 +164,9 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         for (uint32_t t = 0; t < L.H; ++t) perm[t] = t;
         if (L.bt != NO_TABLE) {
             const uint32_t n = L.H - 1;
-            std::vector<double> key(n);
-            for (uint32_t t = 0; t < n; ++t) key[t] = psi_of(i128_to_double(val[t].re), i128_to_double(val[t].im));
+            std::vector<float> key(n);  // FP32 keys, as the device sort stores them
+            for (uint32_t t = 0; t < n; ++t)
+                key[t] = (float)psi_of(i128_to_double(val[t].re), i128_to_double(val[t].im));
             std::sort(perm.begin(), perm.begin() + n, [&](uint32_t x, uint32_t y) {
                 return key[x] < key[y] || (key[x] == key[y] && x < y);
             });
