@@ -149,3 +149,25 @@ def test_autotune_keeps_counts(hss_mod, ctx_big):
         assert tot == 10_369_431_230  # evidence/benchmarks/v45-ab.json:11
     finally:
         ctx_big.set_tuning(**old)
+
+
+def test_largest_tables_window_vs_plain(hss_mod):
+    """Tables at the largest N_max (2^26): positions at LIMIT = 2^26 (partner
+    indices k up to 2^26, the widest packed queue ids, the largest half-sets)
+    give the same per-position counts with window mode forced everywhere and
+    switched off, and the same hits (none expected this high)."""
+    ctx = hss_mod.Context(1 << 26)
+    try:
+        limit = 1 << 26
+        roots = [limit - 1, limit - 7, 66_000_000, 65_536_001]
+        with tuned(ctx, **MODES["everywhere"]):
+            r_win = ctx.search(limit, roots)
+        with tuned(ctx, **MODES["off"]):
+            r_off = ctx.search(limit, roots)
+        print("2^26", r_win.candidates, r_win.stats["window_tiles"], flush=True)
+        assert r_win.stats["window_tiles"] > 0
+        assert r_win.candidates == r_off.candidates and r_win.leaves == r_off.leaves
+        assert r_win.pairs == r_off.pairs and r_win.hits == r_off.hits
+        assert sum(r_win.candidates) > 0
+    finally:
+        ctx.close()
