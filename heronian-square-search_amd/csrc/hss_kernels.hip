@@ -825,7 +825,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
                 const uint32_t uend = liveu ? (uint32_t)Q.en[pc] : 0u;
                 const bool cj = pc >= 4;
-                const double qc = Q.c, qd = Q.d;
+                // u*conj(v) is u*v with d negated: one sign per unit, none per leaf
+                const double qc = Q.c, qd = cj ? -Q.d : Q.d;
                 const uint32_t qr = Q.cres;
                 const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
                 const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
@@ -853,8 +854,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const double t1 = u.im * qd, t2 = u.re * qd;
                     const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
                     const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
-                    UX[s] = __builtin_fma(u.re, qc, cj ? t1 : -t1);
-                    UY[s] = __builtin_fma(u.im, qc, cj ? -t2 : t2);
+                    UX[s] = __builtin_fma(u.re, qc, -t1);
+                    UY[s] = __builtin_fma(u.im, qc, t2);
                     const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
                     const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
                     UF[s] = leaf_decide(live, UX[s], UY[s], vXs, vYs, qx, cnt);

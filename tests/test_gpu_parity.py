@@ -95,6 +95,20 @@ def test_seeded_roots_vs_oracle(ctx_small, oracle):
             assert (c, lv, pr) == (oc, olv, ev), (limit, l)
 
 
+def test_empty_and_edge_positions(ctx_small, oracle):
+    """No roots: an empty result.  Edge positions: l = 1 (the whole partner
+    range), l = LIMIT (one partner m = l) and l = LIMIT - 1, against the oracle."""
+    r = ctx_small.search(5000, [])
+    assert r.total == 0 and r.candidates == [] and r.hits == []
+    for limit in (5000, 70000):
+        roots = [1, limit - 1, limit]
+        got = ctx_small.search(limit, roots)
+        for l, c, lv, pr in zip(roots, got.candidates, got.leaves, got.pairs):
+            oc, _ = oracle.position(limit, l, nmax=limit)
+            ev, olv = oracle.work(limit, l, nmax=limit)
+            assert (c, lv, pr) == (oc, olv, ev), (limit, l)
+
+
 def test_order_duplicates_and_split(ctx_small):
     roots = [4427, 3, 1853, 3, 4999, 1, 2, 4380]
     a = ctx_small.search(5000, roots)
