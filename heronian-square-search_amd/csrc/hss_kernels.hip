@@ -8,7 +8,7 @@
 //                     fixed(i) * prod pi_p^r conj(pi_p)^(4e_p - r) whose digit
 //                     tuples are lexicographically <= the central tuple; FP64
 //                     coordinates (exact: |coord| <= i^2 < 2^53) and residues mod
-//                     Q1.  The last entry of each half-set is the real one.
+//                     180 (QA).  The last entry of each half-set is the real one.
 //   pval              per split prime p and merged exponent E <= 2*floor(log_p N):
 //                     pi^r conj(pi)^(4E-r), r = 0..4E, exact i128
 //
@@ -18,16 +18,18 @@
 // free/central/raw lists per integer (3w+1)/2 entries; this cache keeps only
 // the half-set (w+1)/2 and takes conjugates on the fly.
 //
-// The search kernel (cell_kernel) replaces search_kernel
-// (src/gpu_search_v42.cu:292-345): a persistent grid pulls tiles
-// (left set x range of partner indices) from a device work queue; each
-// wavefront maps one partner half-set entry v per lane and walks the left set
-// u uniformly (scalar loads), evaluating the cells u*v and u*conj(v) with the
-// FP64/residue fast path of hss_math.h, which decides every candidate count
-// exactly.  Counted terms whose zz could be a fourth power are compacted with
-// ballot/mbcnt into a per-wave LDS queue and root-tested 64 at a time; the few
-// survivors, and leaves left undecided by the FP64 band, go to a global buffer
-// that the exact kernel (k_exact) settles with 128/256-bit integer arithmetic.
+// The search kernels replace search_kernel (src/gpu_search_v42.cu:292-345):
+// persistent grids pull tiles (left set x range of partner indices) from
+// device work queues.  In cell_kernel each wavefront maps one partner
+// half-set entry v per lane and walks the left set u uniformly (scalar loads),
+// evaluating the cells u*v and u*conj(v) with the FP64/residue fast path of
+// hss_math.h, which decides every candidate count exactly; cell_win_kernel
+// evaluates only the left entries inside each partner's angle window (left
+// sets sorted by angle).  Leaves with counted terms are compacted with
+// ballot/mbcnt into a per-wave LDS ring and root-tested a wave of terms at a
+// time; the few survivors, and leaves left undecided by the FP64 band, go to a
+// global buffer that the exact kernel (k_exact) settles with 128/256-bit
+// integer arithmetic.
 #include <hip/hip_runtime.h>
 
 #include "hss_build.h"

@@ -22,8 +22,10 @@
 //     below tau/16; an estimate within tau of a bound is "undecided" and the
 //     whole leaf goes to the exact stage;
 //   * a counted term's zz is estimated in FP64 (relative error < 2^-49 because
-//     zz >= m^4); unless its fourth root lies within ROOT_EPS of an integer it
-//     cannot be n^4, so only those terms reach the exact stage.
+//     zz >= m^4); unless it is that close to a fourth power n^4 it cannot be
+//     one, so only such terms reach the exact stage (root_near_integer: on the
+//     host a fourth root within ROOT_EPS of an integer, on the device the
+//     single candidate n with |n^4 - zz| <= 2^-47 zz).
 // The exact stage (exact_leaf) restates the reference chain with 128-bit
 // integers and is the only place where residues mod 16/625/9/7/13/17, the
 // integer fourth root and the acceptance rules are evaluated.
@@ -49,6 +51,7 @@ static constexpr uint32_t QA = 180;        // area residue modulus 4 * 9 * 5
 // left values possibly rounded] + 2^-53 zz [subtraction] < 2^-49.3 zz, so the
 // fourth root is off by < 2^-51.3 n plus two sqrt roundings: < 3e-8 for
 // n <= 2^26.  1e-6 keeps a 30x margin; random zz pass with probability 2e-6.
+// (Host test; the device form in root_near_integer has its own bound.)
 static constexpr double ROOT_EPS = 1e-6;
 
 // ------------------------------------------------------------------ residues
