@@ -729,8 +729,6 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 //     evaluates the unit's leaves with leaf_fast and vector loads of u.
 // Leaves outside the ranges count 0 exactly and are skipped.
 
-static constexpr uint32_t WIN_UNIT = 4;
-
 // 104 bytes: four blocks of four waves fit a CU's LDS with the root queues.
 // The narrowed range bounds are derived from the widened ones (n = w -+ tau),
 // the area-test constants from the class of h (3, 9, 15, 45).
@@ -753,7 +751,14 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
     __shared__ uint2 s_qid[WAVES][QCAP + 1];
     __shared__ WinRec s_rec[WAVES][64];
     __shared__ uint32_t s_mark[WAVES][65];
+    __shared__ uint2 s_area[4];  // (inv24, thr22) per class of h
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x < 4) {
+        const uint32_t i = threadIdx.x;
+        s_area[i] = make_uint2(i == 0 ? AREA_INV[0] : i == 1 ? AREA_INV[1] : i == 2 ? AREA_INV[2] : AREA_INV[3],
+                               i == 0 ? AREA_THR[0] : i == 1 ? AREA_THR[1] : i == 2 ? AREA_THR[2] : AREA_THR[3]);
+    }
+    __syncthreads();
     RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
     WinRec* rec = s_rec[wave];
     uint32_t* mark = s_mark[wave];
@@ -821,7 +826,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
                 const bool liveu = wi < nunits;
                 const uint32_t wq = liveu ? wi - first : 0u;
-                const WinRec& Q = rec[o];
+                const WinRec& Q = *(const WinRec*)((const char*)rec + __umul24(o, (uint32_t)sizeof(WinRec)));
                 uint32_t pc = 0;
                 for (int p = 1; p < 8; ++p) pc += Q.ucum[p] <= wq ? 1u : 0u;
                 const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
@@ -840,18 +845,21 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 qx.h1_lo_n = qx.h1_lo_w + qx.tau;
                 qx.h1_hi_n = qx.h1_hi_w - qx.tau;
                 qx.h2_n = qx.h2_w - qx.tau;
-                qx.inv24 = qh == 0u ? AREA_INV[0] : qh == 1u ? AREA_INV[1] : qh == 2u ? AREA_INV[2] : AREA_INV[3];
-                qx.thr22 = qh == 0u ? AREA_THR[0] : qh == 1u ? AREA_THR[1] : qh == 2u ? AREA_THR[2] : AREA_THR[3];
+                const uint2 ar = s_area[qh];
+                qx.inv24 = ar.x;
+                qx.thr22 = ar.y;
                 const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
                 const uint32_t jpk = (qr >> 16) << ID_U_BITS;
                 double UX[WIN_UNIT], UY[WIN_UNIT];
-                uint32_t UF[WIN_UNIT], UI[WIN_UNIT];
+                uint32_t UF[WIN_UNIT];
+                // the unit's entries from one base: ui0 <= nsort, and the left
+                // entries carry WIN_UNIT entries of padding, so every load is in
+                // bounds; entries past uend are masked by live
+                const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
 #pragma unroll
                 for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-                    const uint32_t ui1 = ui0 + s;
-                    const bool live = ui1 < uend;
-                    const uint32_t ui = live ? ui1 : 0u;  // in-bounds for every lane (uend <= nsort)
-                    const LeftFast u = lf[ui];
+                    const bool live = ui0 + s < uend;
+                    const LeftFast u = up[s];
                     const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
                     const double t1 = u.im * qd, t2 = u.re * qd;
                     const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
@@ -861,10 +869,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
                     const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
                     UF[s] = leaf_decide(live, UX[s], UY[s], vXs, vYs, qx, cnt);
-                    UI[s] = ui;
                 }
 #pragma unroll
-                for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, UI[s] | jpk);
+                for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk);
             }
             __builtin_amdgcn_wave_barrier();
         });

@@ -194,29 +194,38 @@ HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, 
 }
 
 // can an FP64 estimate zz (relative error < 2^-49, zz >= 1) be an exact fourth power?
-HSS_HD inline bool root_near_integer(double zz) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    // zz^(1/4) from two hardware reciprocal square roots (zz^-1/2, then its
-    // own, each within 2^-22 relative even on a pessimistic reading of their
-    // 2^29-ulp bound), refined by two Newton steps on r^4 = zz with the
-    // derivative from the first estimate (1/(4 r^3) ~ r t^2 / 4, t = zz^-1/2):
-    // a step takes relative error e to ~2.5 e^2 + 2^-21 e, so r is within
-    // 2^-38 of zz^(1/4) (0.001 absolute for n <= 2^26) and rint(r) is the only
-    // candidate n.  Then n^4 - zz (n^2 exact, one rounding in the fma) must be
-    // within 2^-47 zz: a true fourth power Z = n^4 has |zz - Z| < 2^-49.3 Z,
-    // and a random zz passes with probability ~2^-48 n.
-    const double t = __builtin_amdgcn_rsq(zz);
-    double r = __builtin_amdgcn_rsq(t);
+// One Newton step on r^4 = zz from r ~ zz^(1/4) and t ~ zz^(-1/2), with the
+// derivative 1/(4 r^3) taken as r t^2 / 4.  With relative errors e of r and
+// e_t of t the result is off by 2.5 e^2 + 2 e e_t + O(e^3) plus a few
+// roundings (r'/rho = (1+e) [1 - (1+e_t)^2 ((1+e)^4 - 1)/4]).
+HSS_HD inline double root4_refine(double zz, double t, double r) {
     const double w = 0.25 * (t * t);
-    for (int i = 0; i < 2; ++i) {
-        const double r2 = r * r;
-        const double d = __builtin_fma(r2, r2, -zz);  // r^4 - zz
-        r = __builtin_fma(-d, r * w, r);
-    }
+    const double r2 = r * r;
+    const double d = __builtin_fma(r2, r2, -zz);  // r^4 - zz
+    return __builtin_fma(-d, r * w, r);
+}
+
+// Is zz within 2^-47 zz of the fourth power of n = rint(r)?  n^2 is exact
+// (n <= 2^26) and the fma rounds once.  A true fourth power Z = n^4 estimated
+// with |zz - Z| < 2^-49.3 Z passes; a random zz passes with probability
+// ~2^-48 n.
+HSS_HD inline bool root_candidate_ok(double zz, double r) {
     const double n = __builtin_rint(r);
     const double n2 = n * n;
     const double e = __builtin_fma(n2, n2, -zz);
     return __builtin_fabs(e) <= zz * 0x1p-47;
+}
+
+HSS_HD inline bool root_near_integer(double zz) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // zz^(1/4) from two hardware reciprocal square roots (t = zz^-1/2, then
+    // t^-1/2), each within 2^-22 relative on a pessimistic reading of their
+    // 2^29-ulp bound, so r starts within 1.5 * 2^-22; one root4_refine step
+    // leaves 2.5 (1.5 * 2^-22)^2 + 3 * 2^-44 < 2^-40.8, i.e. < 4e-5 absolute
+    // for n <= 2^26, and rint(r) is the only candidate n.  (Even with the
+    // hardware bound 2^6 times worse the step stays within 0.2 absolute.)
+    const double t = __builtin_amdgcn_rsq(zz);
+    return root_candidate_ok(zz, root4_refine(zz, t, __builtin_amdgcn_rsq(t)));
 #else
     double r = __builtin_sqrt(__builtin_sqrt(zz));
     double f = r - __builtin_rint(r);

@@ -497,7 +497,7 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     HIP_TRY(B.sorted.reserve(std::max<size_t>(nsorted, 1) * 4));
     HIP_TRY(B.btab.reserve(std::max<size_t>(nsorted, 1) * (WIN_NB + 1) * 4));
     HIP_TRY(B.perm.reserve(entries * 4));
-    HIP_TRY(B.lfast.reserve(entries * sizeof(LeftFast)));
+    HIP_TRY(B.lfast.reserve((entries + WIN_UNIT) * sizeof(LeftFast)));  // padding read by window units
     HIP_TRY(B.lre.reserve(entries * sizeof(i128)));
     HIP_TRY(B.lim.reserve(entries * sizeof(i128)));
     HIP_TRY(B.misc.reserve(64));

@@ -15,6 +15,10 @@ static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
 static constexpr uint32_t WIN_NB = 1024;
 static constexpr uint32_t WIN_MAX_SORT = 4096;
 static constexpr uint32_t WIN_MIN_SORT = 8;
+// Window-mode leaves are evaluated in units of WIN_UNIT consecutive sorted
+// entries; a unit's loads may run up to WIN_UNIT - 1 entries past the end of
+// its left set, so the left-entry arrays carry WIN_UNIT entries of padding.
+static constexpr uint32_t WIN_UNIT = 4;
 static constexpr uint32_t NO_TABLE = 0xFFFFFFFFu;
 // Leaves queued for the root test are named by (entry u, partner entry within
 // its half-set, k, conjugate, counted terms) packed into 64 bits: u below

@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 #include <set>
 #include <string>
 #include <tuple>
@@ -334,6 +335,17 @@ uint64_t hc_area_mismatches(void) {
                 }
     return bad;
 }
+
+// The device's fourth-root candidate with its two hardware reciprocal square
+// roots replaced by correctly rounded ones carrying relative errors dt, dr:
+// the refined r (root4_refine), and whether root_candidate_ok accepts it.
+double hc_root_refined(double zz, double dt, double dr) {
+    const double t = (1.0 / std::sqrt(zz)) * (1.0 + dt);
+    const double r = (1.0 / std::sqrt(t)) * (1.0 + dr);
+    return root4_refine(zz, t, r);
+}
+
+int hc_root_candidate_ok(double zz, double r) { return root_candidate_ok(zz, r) ? 1 : 0; }
 
 // Mean range-window fraction of root l over partners q*k, k in [k0, k1].
 double hc_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1) {

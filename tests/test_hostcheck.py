@@ -5,6 +5,7 @@ and exact stage must reproduce the reference's counts and hits."""
 from __future__ import annotations
 
 import ctypes
+import random
 import subprocess
 
 import pytest
@@ -245,6 +246,42 @@ def test_area_test_is_divisibility(hc):
     exactly the residues divisible by 4h, h = (3 or 9) * (1 or 5)."""
     hc.hc_area_mismatches.restype = ctypes.c_uint64
     assert hc.hc_area_mismatches() == 0
+
+
+def test_device_root_candidate(hc):
+    """The device fourth-root test: two reciprocal square roots, one Newton step
+    (root4_refine), then the single candidate n = rint(r) checked by
+    |n^4 - zz| <= 2^-47 zz.  With reciprocal-square-root errors of 2^-22 each
+    way and zz within 2^-49.3 of n^4 (the fast path's error bound), r stays
+    within 1e-3 of n for n up to 2^26 and the check accepts; zz between fourth
+    powers, or 2^-45 off n^4, is rejected.  Errors of 2^-16 still leave r
+    within 0.25 of n (the refinement is quadratic)."""
+    ref = hc.hc_root_refined
+    ref.restype = ctypes.c_double
+    ref.argtypes = [ctypes.c_double] * 3
+    ok = hc.hc_root_candidate_ok
+    ok.restype = ctypes.c_int
+    ok.argtypes = [ctypes.c_double] * 2
+    rng = random.Random(20261203)
+    ns = [1, 2, 3, 7, 1000, 65535, 2**20 + 1, 9_999_991, 10**7, 2**26 - 1, 2**26]
+    ns += [rng.randrange(1, 2**26 + 1) for _ in range(1500)]
+    errs = (-2.0**-22, 0.0, 2.0**-22)
+    worst = 0.0
+    for n in ns:
+        z4 = float(n**4)
+        for zz in (z4, z4 * (1 + 2.0**-49.3), z4 * (1 - 2.0**-49.3)):
+            for dt in errs:
+                for dr in errs:
+                    r = ref(zz, dt, dr)
+                    worst = max(worst, abs(r - n))
+                    assert ok(zz, r), (n, zz, dt, dr, r)
+        far = ref(z4, 2.0**-16, -2.0**-16)
+        assert abs(far - n) < 0.25, (n, far)
+        between = float((n + 0.5)**4)
+        assert not ok(between, ref(between, 0.0, 0.0))
+        off = z4 * (1 + 2.0**-45)
+        assert not ok(off, ref(off, 0.0, 0.0)), n
+    assert worst < 1e-3, worst
 
 
 def test_window_fraction_bounds(hc):
