@@ -567,11 +567,10 @@ __device__ inline void ctx_from_k(const KCtx& r, LaneCtx& c) {
 }
 
 __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, uint32_t jj, uint32_t Hj,
-                                    uint32_t kdef, Partner& v) {
-    const uint32_t kjq = act ? kj : kdef;  // in-bounds index for every lane (act: kj <= k1)
+                                    uint32_t oj, Partner& v) {
     v.kj = kj;
     v.jj = jj;
-    v.vidx = act ? P.off[kjq] + jj : 0u;
+    v.vidx = act ? oj + jj : 0u;
     v.vc = jj + 1 == Hj;
     v.c = P.cre[v.vidx];
     v.d = P.cim[v.vidx];
@@ -600,11 +599,12 @@ __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const Ti
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
 // (k = kb + 2 lane) are filtered (on_chunk(k) is called with each lane's k
 // first) and their half-set entries flattened; f is called for every 64
-// flattened entries with the lane's entry (act, kj, jj, Hj) and the lane j of
-// its k.
+// flattened entries with the lane's entry (act, kj, jj, Hj), the lane j of its
+// k and the offset oj of k's half-set.  pairs and leaves count the pairs that
+// reach the enumeration and their leaves.
 template <typename C, typename F>
 __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T, const LeftDesc& L, const PosDesc& S,
-                                          uint32_t* mark, uint64_t& pairs, C&& on_chunk, F&& f) {
+                                          uint32_t* mark, uint64_t& pairs, uint64_t& leaves, C&& on_chunk, F&& f) {
     const uint32_t lane = __lane_id();
     for (uint32_t par = 0; par < 2; ++par) {
         const uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
@@ -614,9 +614,12 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
             bool ok = k <= T.k1 && !(S.l_even && !(k & 1u));
             for (uint32_t j = 0; j < S.nsplit; ++j) ok = ok && !divides(k, S.split_inv[j], S.split_thr[j]);
             for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
-            uint32_t Hk = P.off[kq + 1] - P.off[kq];
+            const uint32_t ok0 = P.off[kq];
+            uint32_t Hk = P.off[kq + 1] - ok0;
             Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
             pairs += Hk ? 1 : 0;
+            // leaves of the pair: 2H - 1 per non-central entry of k, H for its central one
+            leaves += Hk ? (uint64_t)Hk * (2u * L.H - 1u) - (L.H - 1u) : 0ull;
             on_chunk(k);
             const uint32_t incl = wave_incl_scan(Hk);
             const uint32_t excl = incl - Hk;
@@ -627,11 +630,12 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
                 const uint32_t j = step_owner(mark, excl, Hk, e0, carry);
                 const uint32_t Hj = (uint32_t)__shfl((int)Hk, (int)j, 64);
                 const uint32_t xj = (uint32_t)__shfl((int)excl, (int)j, 64);
+                const uint32_t oj = (uint32_t)__shfl((int)ok0, (int)j, 64);  // the owner's half-set offset
                 const uint32_t jj = e - xj;
                 // an entry beyond the owner's run never happens; the test keeps
                 // every index in bounds regardless
                 const bool act = e < total && jj < Hj;
-                f(act, kb + 2u * j, act ? jj : 0u, Hj, j);
+                f(act, kb + 2u * j, act ? jj : 0u, Hj, j, oj);
             }
         }
     }
@@ -688,12 +692,11 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             kctx[lane] = KCtx{c.h1_lo_w, c.h1_hi_w, c.h2_w, c.tau, c.inv24, c.thr22};
             __builtin_amdgcn_wave_barrier();
         };
-        for_partner_chunks(P, T, L, S, mark, pairs, on_chunk,
-                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j) {
+        for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
+                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j, uint32_t oj) {
             Partner v;
-            load_partner(P, act, kj, jj, Hj, T.k0, v);
+            load_partner(P, act, kj, jj, Hj, oj, v);
             ctx_from_k(kctx[j], v.ctx);
-            leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
             for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
@@ -774,13 +777,12 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const uint32_t nsort = L.H - 1;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        for_partner_chunks(P, T, L, S, mark, pairs, [](uint32_t) {},
-                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t) {
+        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [](uint32_t) {},
+                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t, uint32_t oj) {
             // ---- phase 1: lane = partner entry
             Partner v;
-            load_partner(P, act, kj, jj, Hj, T.k0, v);
+            load_partner(P, act, kj, jj, Hj, oj, v);
             pair_ctx(L, S, kj, v.ctx);
-            leaves += act ? (v.vc ? L.H : 2ull * L.H - 1) : 0ull;
             central_leaf(P, rq, tc, act, lf[nsort], nsort, v, cnt);
             // ranges of non-central entries
             const uint64_t m = (uint64_t)L.q * kj;

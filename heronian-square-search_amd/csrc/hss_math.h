@@ -365,15 +365,24 @@ struct Window {
     int n;
 };
 
+// 1/x for a positive normal x: on the device the hardware reciprocal (2^29 ulp,
+// i.e. within 2^-23 relative) refined by one Newton step (error ~2^-46 plus a
+// rounding); on the host the division.
+HSS_HD inline double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 HSS_HD inline void make_window(double lo, double hi, double tau, double R, Window& w) {
-    double s0, s1;
-    if (lo <= tau) {
-        s0 = 0.0;
-        s1 = ((hi > -lo ? hi : -lo) + tau) / R;
-    } else {
-        s0 = (lo - tau) / R;
-        s1 = (hi + tau) / R;
-    }
+    // one reciprocal instead of a division per bound (relative error < 1e-13)
+    const double inv = recip(R);
+    const bool low = lo <= tau;
+    double s0 = low ? 0.0 : (lo - tau) * inv;
+    double s1 = ((low ? (hi > -lo ? hi : -lo) : hi) + tau) * inv;
     s0 -= 1e-6;
     s1 += 1e-6;
     s0 = s0 < 0.0 ? 0.0 : (s0 > 1.0 ? 1.0 : s0);
@@ -416,53 +425,82 @@ HSS_HD inline void bucket_range(const uint32_t* bt, double x, double y, uint32_t
 template <uint32_t NB>
 HSS_HD inline void cyclic_ranges(const uint32_t* bt, double a, double b, double shift, uint32_t& s0, uint32_t& e0,
                                  uint32_t& s1, uint32_t& e1) {
+    // x = a + shift reduced modulo pi/2 (floor through the reciprocal: an
+    // off-by-one floor next to a multiple of pi/2 lands within an ulp of 0 or
+    // of pi/2, which the clamp maps to 0, the same point of the circle)
+    constexpr double INV_HALF_PI = 0.63661977236758134;  // 2 / pi
     double x = a + shift;
-    x -= HALF_PI * floor(x / HALF_PI);
+    x -= HALF_PI * floor(x * INV_HALF_PI);
     x = x < 0.0 ? 0.0 : (x >= HALF_PI ? 0.0 : x);
     const double y = x + (b - a);
-    if (y < HALF_PI) {
-        bucket_range<NB>(bt, x, y, s0, e0);
-        s1 = e1 = 0;
-    } else {
-        bucket_range<NB>(bt, x, HALF_PI, s0, e0);
-        bucket_range<NB>(bt, 0.0, y - HALF_PI, s1, e1);
-    }
+    // [x, y] when y < pi/2, else [x, pi/2] and [0, y - pi/2]: bt[bucket(0)] =
+    // bt[0] = 0 and bt[bucket(pi/2) + 1] = bt[NB] = all sorted entries, so two
+    // table reads cover both cases
+    const bool wrap = y >= HALF_PI;
+    const uint32_t t = bt[psi_bucket<NB>(wrap ? y - HALF_PI : y) + 1];
+    s0 = bt[psi_bucket<NB>(x)];
+    e0 = wrap ? bt[NB] : t;
+    s1 = 0;
+    e1 = wrap ? t : 0u;
 }
 
 // Sort four ranges by start and merge overlapping ones in place; merged-away
 // and empty ranges end as [0, 0).  Returns the number of entries covered.
+// The ranges are held in scalar locals (array elements picked by a select
+// would become indexed register reads).
+HSS_HD inline void range_cswap(uint32_t& sa, uint32_t& ea, uint32_t& sb, uint32_t& eb) {
+    const bool sw = sb < sa;
+    const uint32_t s_lo = sw ? sb : sa, e_lo = sw ? eb : ea, s_hi = sw ? sa : sb, e_hi = sw ? ea : eb;
+    sa = s_lo;
+    ea = e_lo;
+    sb = s_hi;
+    eb = e_hi;
+}
+
+// with sa <= sb: fold range b into range a when they touch (b is cleared)
+HSS_HD inline void range_fold(uint32_t sa, uint32_t& ea, uint32_t& sb, uint32_t& eb) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    const bool ov = sa != NONE && sb != NONE && sb <= ea;
+    ea = ov ? (eb > ea ? eb : ea) : ea;
+    sb = ov ? NONE : sb;
+    eb = ov ? NONE : eb;
+}
+
 HSS_HD inline uint32_t merge4(uint32_t* s, uint32_t* e) {
-    for (int i = 0; i < 4; ++i)
-        if (s[i] >= e[i]) s[i] = e[i] = 0xFFFFFFFFu;  // empties sort last
-    auto cs = [&](int i, int j) {
-        const bool sw = s[j] < s[i];
-        const uint32_t ts = sw ? s[j] : s[i], te = sw ? e[j] : e[i];
-        s[j] = sw ? s[i] : s[j];
-        e[j] = sw ? e[i] : e[j];
-        s[i] = ts;
-        e[i] = te;
-    };
-    cs(0, 1);
-    cs(2, 3);
-    cs(0, 2);
-    cs(1, 3);
-    cs(1, 2);
-    for (int i = 1; i < 4; ++i)
-        for (int j = 0; j < i; ++j) {
-            const bool live = s[j] != 0xFFFFFFFFu && s[i] != 0xFFFFFFFFu;
-            const bool ov = live && s[i] <= e[j];
-            e[j] = ov ? (e[i] > e[j] ? e[i] : e[j]) : e[j];
-            s[i] = ov ? 0xFFFFFFFFu : s[i];
-            e[i] = ov ? 0xFFFFFFFFu : e[i];
-        }
-    uint32_t n = 0;
-    for (int i = 0; i < 4; ++i) {
-        const bool empty = s[i] == 0xFFFFFFFFu;
-        n += empty ? 0u : e[i] - s[i];
-        s[i] = empty ? 0u : s[i];
-        e[i] = empty ? 0u : e[i];
-    }
-    return n;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3], e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+    // empties sort last
+    const bool z0 = s0 >= e0, z1 = s1 >= e1, z2 = s2 >= e2, z3 = s3 >= e3;
+    s0 = z0 ? NONE : s0;
+    e0 = z0 ? NONE : e0;
+    s1 = z1 ? NONE : s1;
+    e1 = z1 ? NONE : e1;
+    s2 = z2 ? NONE : s2;
+    e2 = z2 ? NONE : e2;
+    s3 = z3 ? NONE : s3;
+    e3 = z3 ? NONE : e3;
+    range_cswap(s0, e0, s1, e1);
+    range_cswap(s2, e2, s3, e3);
+    range_cswap(s0, e0, s2, e2);
+    range_cswap(s1, e1, s3, e3);
+    range_cswap(s1, e1, s2, e2);
+    // starts are sorted: a range touching an earlier live one joins the first such
+    range_fold(s0, e0, s1, e1);
+    range_fold(s0, e0, s2, e2);
+    range_fold(s1, e1, s2, e2);
+    range_fold(s0, e0, s3, e3);
+    range_fold(s1, e1, s3, e3);
+    range_fold(s2, e2, s3, e3);
+    const bool d0 = s0 == NONE, d1 = s1 == NONE, d2 = s2 == NONE, d3 = s3 == NONE;
+    s[0] = d0 ? 0u : s0;
+    e[0] = d0 ? 0u : e0;
+    s[1] = d1 ? 0u : s1;
+    e[1] = d1 ? 0u : e1;
+    s[2] = d2 ? 0u : s2;
+    e[2] = d2 ? 0u : e2;
+    s[3] = d3 ? 0u : s3;
+    e[3] = d3 ? 0u : e3;
+    return (d0 ? 0u : e0 - s0) + (d1 ? 0u : e1 - s1) + (d2 ? 0u : e2 - s2) + (d3 ? 0u : e3 - s3);
 }
 
 // Entry ranges (up to four, merged) of a left set in window mode for the
