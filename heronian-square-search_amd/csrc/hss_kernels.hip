@@ -32,6 +32,8 @@
 // integer arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hss_build.h"
 #include "hss_kernels.h"
 
@@ -491,7 +493,9 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 // for lane data, with area residues pre-shifted (vXs = vX << 8).  Adds the
 // decided count to cnt and returns the leaf's queue flags: its counted terms
 // (bits 0-3) and 16 when the FP64 band left it undecided; 0 when there is
-// nothing to queue.
+// nothing to queue.  ONE: the pair has h2_w < 0 (tile_one_term), so term2 of
+// either orientation is false and its compares are skipped (same result).
+template <bool ONE = false>
 __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t vXs, uint32_t vYs, const LaneCtx& c,
                                        uint32_t& cnt) {
     // bitwise logic on lane masks: no short-circuit branches
@@ -501,9 +505,9 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     // orientation 1 (d = X) and 2 (d = Y): narrowed ranges decide, the widened
     // ones minus the narrowed ones are the undecided band
     const bool t1w = (ax >= c.h1_lo_w) & (ax <= c.h1_hi_w), t1n = (ax >= c.h1_lo_n) & (ax <= c.h1_hi_n);
-    const bool t2w = ax <= c.h2_w, t2n = ax <= c.h2_n;
+    const bool t2w = !ONE && ax <= c.h2_w, t2n = !ONE && ax <= c.h2_n;
     const bool t3w = (ay >= c.h1_lo_w) & (ay <= c.h1_hi_w), t3n = (ay >= c.h1_lo_n) & (ay <= c.h1_hi_n);
-    const bool t4w = ay <= c.h2_w, t4n = ay <= c.h2_n;
+    const bool t4w = !ONE && ay <= c.h2_w, t4n = !ONE && ay <= c.h2_n;
     const double dd = ax - ay;
     const bool eq = __builtin_fabs(dd) <= c.tau;
     const bool und = (a1 & ((t1w & !t1n) | (t2w & !t2n) | (ay <= c.tau))) |
@@ -682,6 +686,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const PosDesc& S = pos[L.pos];
         const TileCtx tc{T.left, L.q, S.l4_d};
         const LeftFast* lf = lfast + L.off;
+        const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0;
         // the pair context once per k (a k's half-set entries share it)
@@ -699,19 +704,27 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             ctx_from_k(kctx[j], v.ctx);
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
-            for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
-                const LeftFast u = lf[ui];
-                const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                const double t1 = u.im * v.d, t2 = u.re * v.d;
-                const uint32_t pac = __umul24(ua, v.c180s), pbd = __umul24(ub, v.d180s);
-                const uint32_t pad = __umul24(ua, v.d180s), pbc = __umul24(ub, v.c180s);
-                const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
-                const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
-                const uint32_t f0 = leaf_decide(act, X0, Y0, pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, cnt);
-                const uint32_t f1 = leaf_decide(act & !v.vc, X1, Y1, pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, cnt);
-                leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
-                leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
-            }
+            // (instantiated for tiles with and without second-sign terms)
+            const auto rows = [&](auto one_c) {
+                constexpr bool ONE = decltype(one_c)::value;
+                for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
+                    const LeftFast u = lf[ui];
+                    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                    const double t1 = u.im * v.d, t2 = u.re * v.d;
+                    const uint32_t pac = __umul24(ua, v.c180s), pbd = __umul24(ub, v.d180s);
+                    const uint32_t pad = __umul24(ua, v.d180s), pbc = __umul24(ub, v.c180s);
+                    const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
+                    const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
+                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, cnt);
+                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, cnt);
+                    leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
+                    leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
+                }
+            };
+            if (one)
+                rows(std::true_type{});
+            else
+                rows(std::false_type{});
             central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
         });
         root_flush(P, rq, tc);
@@ -775,6 +788,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const LeftFast* lf = lfast + L.off;
         const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
         const uint32_t nsort = L.H - 1;
+        const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
         for_partner_chunks(P, T, L, S, mark, pairs, leaves, [](uint32_t) {},
@@ -821,60 +835,68 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             const uint32_t nunits = __builtin_amdgcn_readlane(ucl, 63);
             __builtin_amdgcn_wave_barrier();
             // ---- phase 2: lane = unit of WIN_UNIT consecutive entries of one range
-            uint32_t carry = 0;
-            for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
-                const uint32_t wi = w0 + (uint32_t)lane;
-                const uint32_t o = step_owner(mark, uex, units, w0, carry);
-                const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
-                const bool liveu = wi < nunits;
-                const uint32_t wq = liveu ? wi - first : 0u;
-                const WinRec& Q = *(const WinRec*)((const char*)rec + __umul24(o, (uint32_t)sizeof(WinRec)));
-                uint32_t pc = 0;
-                for (int p = 1; p < 8; ++p) pc += Q.ucum[p] <= wq ? 1u : 0u;
-                const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
-                const uint32_t uend = liveu ? (uint32_t)Q.en[pc] : 0u;
-                const bool cj = pc >= 4;
-                // u*conj(v) is u*v with d negated: one sign per unit, none per leaf
-                const double qc = Q.c, qd = cj ? -Q.d : Q.d;
-                const uint32_t qr = Q.cres;
-                const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
-                const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
-                LaneCtx qx;
-                qx.h1_lo_w = Q.h1_lo_w;
-                qx.h1_hi_w = Q.h1_hi_w;
-                qx.h2_w = Q.h2_w;
-                qx.tau = Q.tau;
-                qx.h1_lo_n = qx.h1_lo_w + qx.tau;
-                qx.h1_hi_n = qx.h1_hi_w - qx.tau;
-                qx.h2_n = qx.h2_w - qx.tau;
-                const uint2 ar = s_area[qh];
-                qx.inv24 = ar.x;
-                qx.thr22 = ar.y;
-                const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
-                const uint32_t jpk = (qr >> 16) << ID_U_BITS;
-                double UX[WIN_UNIT], UY[WIN_UNIT];
-                uint32_t UF[WIN_UNIT];
-                // the unit's entries from one base: ui0 <= nsort, and the left
-                // entries carry WIN_UNIT entries of padding, so every load is in
-                // bounds; entries past uend are masked by live
-                const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
+            // (instantiated for tiles with and without second-sign terms)
+            const auto phase2 = [&](auto one_c) {
+                constexpr bool ONE = decltype(one_c)::value;
+                uint32_t carry = 0;
+                for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
+                    const uint32_t wi = w0 + (uint32_t)lane;
+                    const uint32_t o = step_owner(mark, uex, units, w0, carry);
+                    const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
+                    const bool liveu = wi < nunits;
+                    const uint32_t wq = liveu ? wi - first : 0u;
+                    const WinRec& Q = *(const WinRec*)((const char*)rec + __umul24(o, (uint32_t)sizeof(WinRec)));
+                    uint32_t pc = 0;
+                    for (int p = 1; p < 8; ++p) pc += Q.ucum[p] <= wq ? 1u : 0u;
+                    const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
+                    const uint32_t uend = liveu ? (uint32_t)Q.en[pc] : 0u;
+                    const bool cj = pc >= 4;
+                    // u*conj(v) is u*v with d negated: one sign per unit, none per leaf
+                    const double qc = Q.c, qd = cj ? -Q.d : Q.d;
+                    const uint32_t qr = Q.cres;
+                    const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
+                    const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
+                    LaneCtx qx;
+                    qx.h1_lo_w = Q.h1_lo_w;
+                    qx.h1_hi_w = Q.h1_hi_w;
+                    qx.h2_w = Q.h2_w;
+                    qx.tau = Q.tau;
+                    qx.h1_lo_n = qx.h1_lo_w + qx.tau;
+                    qx.h1_hi_n = qx.h1_hi_w - qx.tau;
+                    qx.h2_n = qx.h2_w - qx.tau;
+                    const uint2 ar = s_area[qh];
+                    qx.inv24 = ar.x;
+                    qx.thr22 = ar.y;
+                    const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
+                    const uint32_t jpk = (qr >> 16) << ID_U_BITS;
+                    double UX[WIN_UNIT], UY[WIN_UNIT];
+                    uint32_t UF[WIN_UNIT];
+                    // the unit's entries from one base: ui0 <= nsort, and the left
+                    // entries carry WIN_UNIT entries of padding, so every load is in
+                    // bounds; entries past uend are masked by live
+                    const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
 #pragma unroll
-                for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-                    const bool live = ui0 + s < uend;
-                    const LeftFast u = up[s];
-                    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
-                    const double t1 = u.im * qd, t2 = u.re * qd;
-                    const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
-                    const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
-                    UX[s] = __builtin_fma(u.re, qc, -t1);
-                    UY[s] = __builtin_fma(u.im, qc, t2);
-                    const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
-                    const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
-                    UF[s] = leaf_decide(live, UX[s], UY[s], vXs, vYs, qx, cnt);
+                    for (uint32_t s = 0; s < WIN_UNIT; ++s) {
+                        const bool live = ui0 + s < uend;
+                        const LeftFast u = up[s];
+                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+                        const double t1 = u.im * qd, t2 = u.re * qd;
+                        const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
+                        const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
+                        UX[s] = __builtin_fma(u.re, qc, -t1);
+                        UY[s] = __builtin_fma(u.im, qc, t2);
+                        const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
+                        const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
+                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, cnt);
+                    }
+#pragma unroll
+                    for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk);
                 }
-#pragma unroll
-                for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk);
-            }
+            };
+            if (one)
+                phase2(std::true_type{});
+            else
+                phase2(std::false_type{});
             __builtin_amdgcn_wave_barrier();
         });
         root_flush(P, rq, tc);

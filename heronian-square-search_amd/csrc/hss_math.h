@@ -180,6 +180,20 @@ HSS_HD inline bool area_ok_s(uint32_t vs, const LaneCtx& c) {
 
 HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) { return area_ok_s(va << 8, c); }
 
+// Do all pairs (l, q k), k >= k0, have lo > tau?  Then h2_w = (tau - lo)/2 < 0
+// and the second-sign terms (term2 of both orientations: d = -|2D|, i.e. zz =
+// l^4 + m^4 + |2D| <= N^4) can neither count nor be undecided for any leaf.
+// lo - tau grows with m (by 4m^3 - 2^-41 l^2 m > 0 per unit), so lo > 2 tau at
+// m0 = q k0 leaves a margin of tau (far above the FP64 roundings of lo and
+// tau) for every larger m.
+HSS_HD inline bool tile_one_term(uint32_t q, uint32_t k0, double l4_d, double l2_d, double n4_d) {
+    const double md = (double)q * (double)k0;  // exact: below 2^26
+    const double m2 = md * md;
+    const double lo = (l4_d + m2 * m2) - n4_d;
+    const double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
+    return lo > 2.0 * tau;
+}
+
 // Range terms of one orientation with d = 2D: term1 counts d = +|2D|, term2
 // d = -|2D|.  undecided is set when the FP64 estimate is within tau of a bound.
 HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, bool& undecided) {

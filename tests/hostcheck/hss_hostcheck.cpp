@@ -347,6 +347,35 @@ double hc_root_refined(double zz, double dt, double dr) {
 
 int hc_root_candidate_ok(double zz, double r) { return root_candidate_ok(zz, r) ? 1 : 0; }
 
+// Partners k in [k0, k1] of root l at limit (signature factor q) whose pair
+// context has h2_w >= 0 although tile_one_term(q, k0) holds: the kernels then
+// skip second-sign terms that could count.  Must be 0.
+// With any = 1: partners in [k0, k1] with h2_w >= 0, whatever the rule says.
+uint64_t hc_one_term_violations(uint64_t l, uint64_t limit, uint32_t q, uint32_t k0, uint32_t k1, int any) {
+    const double nd = (double)(limit * limit), l2 = (double)(l * l);
+    const double n4 = nd * nd, l4 = l2 * l2;
+    if (!any && !tile_one_term(q, k0, l4, l2, n4)) return 0;
+    uint64_t bad = 0;
+    for (uint32_t k = k0; k <= k1; ++k) {
+        LaneCtx c;
+        make_lane_ctx((uint64_t)q * k, l4, l2, n4, 1, 1, 1, 1, c);
+        bad += c.h2_w >= 0.0;
+    }
+    return bad;
+}
+
+// Smallest k with tile_one_term(q, k) for root l at limit (k_max + 1 if none).
+uint32_t hc_one_term_first(uint64_t l, uint64_t limit, uint32_t q, uint32_t k_max) {
+    const double nd = (double)(limit * limit), l2 = (double)(l * l);
+    const double n4 = nd * nd, l4 = l2 * l2;
+    uint32_t lo = 1, hi = k_max + 1;  // tile_one_term is monotone in k
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (tile_one_term(q, mid, l4, l2, n4)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
 // Mean range-window fraction of root l over partners q*k, k in [k0, k1].
 double hc_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0, uint32_t k1) {
     return tile_window_fraction(l, limit, q, k0, k1, 33);

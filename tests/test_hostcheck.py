@@ -284,6 +284,29 @@ def test_device_root_candidate(hc):
     assert worst < 1e-3, worst
 
 
+def test_one_term_tiles(hc):
+    """Tiles whose smallest partner already has lo > 2 tau skip the second-sign
+    range compares (tile_one_term).  From the switch-over partner on, every
+    pair's context has h2_w < 0, so skipping changes no decision; a few
+    partners before it h2_w >= 0 (the rule is tight, not vacuous)."""
+    first = hc.hc_one_term_first
+    first.restype = ctypes.c_uint32
+    first.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+    bad = hc.hc_one_term_violations
+    bad.restype = ctypes.c_uint64
+    bad.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    cases = [(7_500_000, 10_000_000, 1), (7_500_511, 10_000_000, 5), (9_000_001, 10_000_000, 1),
+             (11789, 70_000, 1), (11789, 70_000, 13), (60_000, 70_000, 1), (2**25, 2**26, 1)]
+    for l, limit, q in cases:
+        k_max = limit // q
+        k = first(l, limit, q, k_max)
+        assert k <= k_max, (l, limit, q)
+        for k0 in range(max(1, k - 2), k + 3):
+            assert bad(l, limit, q, k0, min(k_max, k0 + 2000), 0) == 0, (l, limit, q, k0)
+        assert bad(l, limit, q, k, k_max, 1) == 0, (l, limit, q)
+        assert k > 3 and bad(l, limit, q, k - 3, k - 1, 1) > 0, (l, limit, q, k)
+
+
 def test_window_fraction_bounds(hc):
     """The planner's window measure is a fraction of the circle, 1 where the
     range test cannot fail (small m, large LIMIT) and small near LIMIT."""
