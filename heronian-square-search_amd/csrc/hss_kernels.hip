@@ -257,8 +257,8 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
-            f.pad = 0;
+            f.ra = smod128(z.re, QA);
+            f.rb = smod128(z.im, QA);
             lfast[e] = f;
         }
     }
@@ -538,6 +538,7 @@ __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileC
 struct Partner {
     double c, d;
     uint32_t c180s, d180s;  // (c mod 180) << 8, (d mod 180) << 8
+    uint32_t n180s;         // (-d mod 180) << 8
     uint32_t vidx, kj, jj;
     bool vc;                // v is the central (real) entry of its half-set
     LaneCtx ctx;
@@ -581,15 +582,18 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
     const uint32_t vr = P.cres[v.vidx];
     v.c180s = (vr & 0xFFFFu) << 8;
     v.d180s = (vr >> 8) & 0xFFFF00u;
+    v.n180s = v.d180s ? (QA << 8) - v.d180s : 0u;
 }
 
 // The central (real) entry u of the left set against v: leaf u*v only; with a
 // real v too, the real-leaf rule.
 __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
                                     const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
-    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+    const uint32_t ua = u.ra, ub = u.rb;
     const double t1 = u.im * v.d, t2 = u.re * v.d;
-    const uint32_t vXs = __umul24(ua, v.c180s) + (QA * QA << 8) - __umul24(ub, v.d180s);
+    // residues of X = re c - im d and Y = im c + re d with -d as its residue:
+    // sums of two products below 2 * 180^2 < 2^16, no subtraction
+    const uint32_t vXs = __umul24(ua, v.c180s) + __umul24(ub, v.n180s);
     const uint32_t vYs = __umul24(ua, v.d180s) + __umul24(ub, v.c180s);
     const bool real = act && v.vc;
     const bool a2 = real && area_ok_s(vXs, v.ctx);
@@ -709,14 +713,15 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 constexpr bool ONE = decltype(one_c)::value;
                 for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
                     const LeftFast u = lf[ui];
-                    const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
                     const double t1 = u.im * v.d, t2 = u.re * v.d;
-                    const uint32_t pac = __umul24(ua, v.c180s), pbd = __umul24(ub, v.d180s);
-                    const uint32_t pad = __umul24(ua, v.d180s), pbc = __umul24(ub, v.c180s);
+                    // residues (central_leaf): u*v has d, u*conj(v) has -d
+                    const uint32_t pac = __umul24(u.ra, v.c180s), pbc = __umul24(u.rb, v.c180s);
+                    const uint32_t vX0 = pac + __umul24(u.rb, v.n180s), vY0 = pbc + __umul24(u.ra, v.d180s);
+                    const uint32_t vX1 = pac + __umul24(u.rb, v.d180s), vY1 = pbc + __umul24(u.ra, v.n180s);
                     const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
                     const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
-                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, pac + (QA * QA << 8) - pbd, pad + pbc, v.ctx, cnt);
-                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, pac + pbd, pbc + (QA * QA << 8) - pad, v.ctx, cnt);
+                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, cnt);
+                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, cnt);
                     leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
                     leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
                 }
@@ -854,7 +859,10 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     // u*conj(v) is u*v with d negated: one sign per unit, none per leaf
                     const double qc = Q.c, qd = cj ? -Q.d : Q.d;
                     const uint32_t qr = Q.cres;
+                    // residues of the unit's c and d (-d for u*conj(v)) and of minus that d
                     const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
+                    const uint32_t qn180s = qd180s ? (QA << 8) - qd180s : 0u;
+                    const uint32_t qps = cj ? qn180s : qd180s, qns = cj ? qd180s : qn180s;
                     const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
                     LaneCtx qx;
                     qx.h1_lo_w = Q.h1_lo_w;
@@ -879,14 +887,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     for (uint32_t s = 0; s < WIN_UNIT; ++s) {
                         const bool live = ui0 + s < uend;
                         const LeftFast u = up[s];
-                        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
                         const double t1 = u.im * qd, t2 = u.re * qd;
-                        const uint32_t pac = __umul24(ua, qc180s), pbd = __umul24(ub, qd180s);
-                        const uint32_t pad = __umul24(ua, qd180s), pbc = __umul24(ub, qc180s);
                         UX[s] = __builtin_fma(u.re, qc, -t1);
                         UY[s] = __builtin_fma(u.im, qc, t2);
-                        const uint32_t vXs = cj ? pac + pbd : pac + (QA * QA << 8) - pbd;
-                        const uint32_t vYs = cj ? pbc + (QA * QA << 8) - pad : pad + pbc;
+                        const uint32_t vXs = __umul24(u.ra, qc180s) + __umul24(u.rb, qns);
+                        const uint32_t vYs = __umul24(u.rb, qc180s) + __umul24(u.ra, qps);
                         UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, cnt);
                     }
 #pragma unroll

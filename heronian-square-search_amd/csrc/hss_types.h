@@ -76,8 +76,7 @@ struct TileDesc {
 // FP64 view of a left entry used by the hot loop (one scalar load).
 struct LeftFast {
     double re, im;  // FP64 values (exact below 2^53, else correctly rounded)
-    uint32_t r180;  // (re mod 180) | (im mod 180) << 16
-    uint32_t pad;
+    uint32_t ra, rb;  // re mod 180, im mod 180
 };
 
 // Sizes of one batch, read by the kernels from device memory so that the

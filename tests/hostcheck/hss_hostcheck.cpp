@@ -185,8 +185,8 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.r180 = smod128(z.re, QA) | smod128(z.im, QA) << 16;
-            f.pad = 0;
+            f.ra = smod128(z.re, QA);
+            f.rb = smod128(z.im, QA);
             lfast[L.off + t] = f;
         }
     }
@@ -204,7 +204,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         const LeftFast& u = lfast[L.off + ui];
         const double c = T.cre[vidx], d = T.cim[vidx];
         const uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
-        const uint32_t ua = u.r180 & 0xFFFFu, ub = u.r180 >> 16;
+        const uint32_t ua = u.ra, ub = u.rb;
         const double t1 = u.im * d, t2 = u.re * d;
         const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
         ++S.cells;
