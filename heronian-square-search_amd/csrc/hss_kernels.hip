@@ -510,8 +510,14 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     const bool t4w = !ONE && ay <= c.h2_w, t4n = !ONE && ay <= c.h2_n;
     const double dd = ax - ay;
     const bool eq = __builtin_fabs(dd) <= c.tau;
-    const bool und = (a1 & ((t1w & !t1n) | (t2w & !t2n) | (ay <= c.tau))) |
-                     (a2 & ((t3w & !t3n) | (t4w & !t4n) | eq | (ax <= c.tau)));
+    // |Y| <= tau (|X| <= tau) leaves the area of orientation 1 (2) possibly 0.
+    // With ONE that can only matter where the orientation's own range fails:
+    // then l^4 >= lo > 2 tau, and t1w gives |X| <= (l^4 + tau)/2 + tau/16 <=
+    // 0.79 l^4, so |Y| = sqrt(l^4 m^4 - X^2) >= 0.6 l^4 > tau (m >= l); the
+    // same for t3w and |X|.  Those compares are dropped there.
+    const bool und = ONE ? (a1 & t1w & !t1n) | (a2 & ((t3w & !t3n) | eq))
+                         : (a1 & ((t1w & !t1n) | (t2w & !t2n) | (ay <= c.tau))) |
+                               (a2 & ((t3w & !t3n) | (t4w & !t4n) | eq | (ax <= c.tau)));
     const bool ok = !und;
     const bool o2 = a2 & !eq & ok;
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
