@@ -264,70 +264,85 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
     }
 }
 
-// One block per sorted left set (grid-stride over hdr->nsorted): order
-// the non-central entries by psi (angle modulo pi/2, from the same FP64
+// One block per sorted left set (grid-stride over hdr->nsorted): order the
+// non-central entries by psi (angle modulo pi/2, from the same FP64
 // coordinates k_build_left stores), write the permutation and the bucket
-// table bt[b] = number of entries with psi_bucket < b, b = 0..WIN_NB.
+// table bt[b] = number of entries with psi_bucket < b, b = 0..WIN_NB.  A
+// counting sort on the buckets (whose starts are that table) followed by an
+// insertion sort of each bucket by (key, index): entries of one left set share
+// their norm, so their angles are distinct and spread over the buckets (a few
+// per bucket), and the order equals a full sort by (key, index).
 __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const uint32_t* sorted,
                                                    const LeftBuild* builds, const LeftDesc* lefts,
                                                    const uint32_t* vbase, const i128* vre, const i128* vim,
                                                    uint32_t* perm, uint32_t* btab) {
-    __shared__ float key[WIN_MAX_SORT];  // psi_u rounded to FP32 (the window's error budget allows it)
-    __shared__ uint16_t idx[WIN_MAX_SORT];
+    __shared__ float key[WIN_MAX_SORT];       // psi_u rounded to FP32 (the window's error budget allows it)
+    __shared__ uint16_t order[WIN_MAX_SORT];  // entry indices in sorted order
+    __shared__ uint32_t start[WIN_NB + 1];    // bucket counts, then bucket starts
+    __shared__ uint32_t cursor[WIN_NB];
+    __shared__ uint32_t part[256];
+    const uint32_t tid = threadIdx.x;
     const uint32_t nsorted = hdr->nsorted;
     for (uint32_t bi = blockIdx.x; bi < nsorted; bi += gridDim.x) {  // uniform per block
         const uint32_t li = sorted[bi];
         const LeftBuild b = builds[li];
         const LeftDesc L = lefts[li];
         const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
-        uint32_t n2 = 1;
-        while (n2 < n) n2 <<= 1;
-        for (uint32_t t = threadIdx.x; t < n2; t += blockDim.x) {
-            float k = 3e38f;
-            if (t < n) {
-                GInt z = left_value(b, t, vbase, vre, vim);
-                k = (float)psi_of(i128_to_double(z.re), i128_to_double(z.im));
-            }
+        for (uint32_t bk = tid; bk <= WIN_NB; bk += blockDim.x) start[bk] = 0;
+        __syncthreads();
+        for (uint32_t t = tid; t < n; t += blockDim.x) {
+            const GInt z = left_value(b, t, vbase, vre, vim);
+            const float k = (float)psi_of(i128_to_double(z.re), i128_to_double(z.im));
             key[t] = k;
-            idx[t] = (uint16_t)t;
+            atomicAdd(&start[psi_bucket<WIN_NB>(k)], 1u);
         }
         __syncthreads();
-        // bitonic sort of (key, idx) over n2 entries
-        for (uint32_t size = 2; size <= n2; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t t = threadIdx.x; t < n2 / 2; t += blockDim.x) {
-                    const uint32_t i = 2 * t - (t & (stride - 1));
-                    const uint32_t j = i + stride;
-                    const bool up = (i & size) == 0;
-                    const float ki = key[i], kj = key[j];
-                    const bool sw = up ? (ki > kj || (ki == kj && idx[i] > idx[j]))
-                                       : (ki < kj || (ki == kj && idx[i] < idx[j]));
-                    if (sw) {
-                        key[i] = kj;
-                        key[j] = ki;
-                        const uint16_t ti = idx[i];
-                        idx[i] = idx[j];
-                        idx[j] = ti;
-                    }
+        // exclusive scan of the WIN_NB counts, WIN_NB / 256 per thread
+        static_assert(WIN_NB == 4 * 256, "scan layout");
+        uint32_t c[4], sum = 0;
+        for (int j = 0; j < 4; ++j) {
+            c[j] = start[4 * tid + j];
+            sum += c[j];
+        }
+        part[tid] = sum;
+        __syncthreads();
+        for (uint32_t d = 1; d < 256; d <<= 1) {
+            const uint32_t v = tid >= d ? part[tid - d] : 0u;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        uint32_t run = part[tid] - sum;
+        for (int j = 0; j < 4; ++j) {
+            start[4 * tid + j] = run;
+            cursor[4 * tid + j] = run;
+            run += c[j];
+        }
+        if (tid == 0) start[WIN_NB] = n;
+        __syncthreads();
+        for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&cursor[psi_bucket<WIN_NB>(key[t])], 1u)] = (uint16_t)t;
+        __syncthreads();
+        for (uint32_t bk = tid; bk < WIN_NB; bk += blockDim.x) {
+            const uint32_t s0 = start[bk], s1 = start[bk + 1];
+            for (uint32_t i = s0 + 1; i < s1; ++i) {
+                const uint16_t x = order[i];
+                const float kx = key[x];
+                uint32_t j = i;
+                for (; j > s0; --j) {
+                    const uint16_t y = order[j - 1];
+                    const float ky = key[y];
+                    if (ky < kx || (ky == kx && y < x)) break;
+                    order[j] = y;
                 }
-                __syncthreads();
+                order[j] = x;
             }
         }
-        for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) perm[L.off + t] = idx[t];
-        if (threadIdx.x == 0) perm[L.off + n] = n;  // the central entry stays last
+        __syncthreads();
+        for (uint32_t t = tid; t < n; t += blockDim.x) perm[L.off + t] = order[t];
+        if (tid == 0) perm[L.off + n] = n;  // the central entry stays last
         uint32_t* bt = btab + (size_t)L.bt * (WIN_NB + 1);
-        for (uint32_t bk = threadIdx.x; bk <= WIN_NB; bk += blockDim.x) {
-            uint32_t lo = 0, hi = n;  // first t with psi_bucket(key[t]) >= bk
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (psi_bucket<WIN_NB>(key[mid]) < bk)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            bt[bk] = bk == WIN_NB ? n : lo;
-        }
-        __syncthreads();  // key[] is rewritten by the next left set
+        for (uint32_t bk = tid; bk <= WIN_NB; bk += blockDim.x) bt[bk] = start[bk];
+        __syncthreads();  // the arrays are rewritten by the next left set
     }
 }
 

@@ -23,8 +23,9 @@ namespace hss {
 
 // Scheduling knobs of one batch, keyed on Gaussian weight.  A left set of
 // H = (w + 1) / 2 entries (w the Gaussian weight of l with the shared prime
-// powers merged) falls in weight class floor(log2 H) - 3 (H in [9, 4097]:
-// classes 0..8; smaller or larger left sets never use window mode).  A tile
+// powers merged) falls in weight class min(floor(log2 H) - 3, 8) (H in
+// [9, 8193]: classes 0..8; smaller or larger left sets never use window
+// mode).  A tile
 // runs in window mode (cell_win_kernel) when the mean range-window fraction of
 // its partner range is at most win_max_frac[class]; otherwise in plain mode
 // (cell_kernel).  Counts never depend on these values, only speed does.

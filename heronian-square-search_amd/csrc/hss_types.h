@@ -13,7 +13,7 @@ static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
 // sorted by the angle of their entries modulo pi/2 (the central entry stays
 // last) with a table of WIN_NB + 1 bucket starts.
 static constexpr uint32_t WIN_NB = 1024;
-static constexpr uint32_t WIN_MAX_SORT = 4096;
+static constexpr uint32_t WIN_MAX_SORT = 8192;
 static constexpr uint32_t WIN_MIN_SORT = 8;
 // Window-mode leaves are evaluated in units of WIN_UNIT consecutive sorted
 // entries; a unit's loads may run up to WIN_UNIT - 1 entries past the end of
