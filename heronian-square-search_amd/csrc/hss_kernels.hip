@@ -519,9 +519,10 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     const bool a2 = live & area_ok_s(vXs, c);
     // orientation 1 (d = X) and 2 (d = Y): narrowed ranges decide, the widened
     // ones minus the narrowed ones are the undecided band
-    const bool t1w = (ax >= c.h1_lo_w) & (ax <= c.h1_hi_w), t1n = (ax >= c.h1_lo_n) & (ax <= c.h1_hi_n);
+    const double dx = __builtin_fabs(ax - c.mid), dy = __builtin_fabs(ay - c.mid);
+    const bool t1w = dx <= c.half_w, t1n = dx <= c.half_n;
     const bool t2w = !ONE && ax <= c.h2_w, t2n = !ONE && ax <= c.h2_n;
-    const bool t3w = (ay >= c.h1_lo_w) & (ay <= c.h1_hi_w), t3n = (ay >= c.h1_lo_n) & (ay <= c.h1_hi_n);
+    const bool t3w = dy <= c.half_w, t3n = dy <= c.half_n;
     const bool t4w = !ONE && ay <= c.h2_w, t4n = !ONE && ay <= c.h2_n;
     const double dd = ax - ay;
     const bool eq = __builtin_fabs(dd) <= c.tau;
@@ -530,9 +531,10 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     // then l^4 >= lo > 2 tau, and t1w gives |X| <= (l^4 + tau)/2 + tau/16 <=
     // 0.79 l^4, so |Y| = sqrt(l^4 m^4 - X^2) >= 0.6 l^4 > tau (m >= l); the
     // same for t3w and |X|.  Those compares are dropped there.
-    const bool und = ONE ? (a1 & t1w & !t1n) | (a2 & ((t3w & !t3n) | eq))
-                         : (a1 & ((t1w & !t1n) | (t2w & !t2n) | (ay <= c.tau))) |
-                               (a2 & ((t3w & !t3n) | (t4w & !t4n) | eq | (ax <= c.tau)));
+    // (a narrowed range lies inside its widened one: "w and not n" is w != n)
+    const bool und = ONE ? (a1 & (t1w != t1n)) | (a2 & ((t3w != t3n) | eq))
+                         : (a1 & ((t1w != t1n) | (t2w != t2n) | (ay <= c.tau))) |
+                               (a2 & ((t3w != t3n) | (t4w != t4n) | eq | (ax <= c.tau)));
     const bool ok = !und;
     const bool o2 = a2 & !eq & ok;
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
@@ -575,17 +577,16 @@ __device__ inline void pair_ctx(const LeftDesc& L, const PosDesc& S, uint32_t k,
 // area-test constants: one per lane of a partner chunk, shared by the k's
 // half-set entries.
 struct KCtx {
-    double h1_lo_w, h1_hi_w, h2_w, tau;
+    double mid, half_w, h2_w, tau;
     uint32_t inv24, thr22;
 };
 
 __device__ inline void ctx_from_k(const KCtx& r, LaneCtx& c) {
-    c.h1_lo_w = r.h1_lo_w;
-    c.h1_hi_w = r.h1_hi_w;
+    c.mid = r.mid;
+    c.half_w = r.half_w;
     c.h2_w = r.h2_w;
     c.tau = r.tau;
-    c.h1_lo_n = r.h1_lo_w + r.tau;
-    c.h1_hi_n = r.h1_hi_w - r.tau;
+    c.half_n = r.half_w - r.tau;
     c.h2_n = r.h2_w - r.tau;
     c.inv24 = r.inv24;
     c.thr22 = r.thr22;
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             LaneCtx c;
             pair_ctx(L, S, k, c);
             __builtin_amdgcn_wave_barrier();
-            kctx[lane] = KCtx{c.h1_lo_w, c.h1_hi_w, c.h2_w, c.tau, c.inv24, c.thr22};
+            kctx[lane] = KCtx{c.mid, c.half_w, c.h2_w, c.tau, c.inv24, c.thr22};
             __builtin_amdgcn_wave_barrier();
         };
         for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
@@ -776,7 +777,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 // the area-test constants from the class of h (3, 9, 15, 45).
 struct WinRec {
     double c, d;  // partner v
-    double h1_lo_w, h1_hi_w, h2_w, tau;
+    double mid, half_w, h2_w, tau;
     uint32_t cres;    // c mod 180 | (d mod 180) << 8 | partner entry << 16
     uint32_t kjh;     // k | class of h << 28
     uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
@@ -825,8 +826,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             pair_ctx(L, S, kj, v.ctx);
             central_leaf(P, rq, tc, act, lf[nsort], nsort, v, cnt);
             // ranges of non-central entries
-            const uint64_t m = (uint64_t)L.q * kj;
-            const double m2 = (double)(m * m);
+            const double md = (double)L.q * (double)kj;  // exact: m < 2^26
+            const double m2 = md * md;                      // exact: m^2 < 2^52
             Window w;
             make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, v.ctx.tau, 2.0 * S.l2_d * m2, w);
             const double pv = psi_of_f(v.c, v.d);
@@ -850,8 +851,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             const uint32_t hcl = inv == AREA_INV[0] ? 0u : inv == AREA_INV[1] ? 1u : inv == AREA_INV[2] ? 2u : 3u;
             R.c = v.c;
             R.d = v.d;
-            R.h1_lo_w = v.ctx.h1_lo_w;
-            R.h1_hi_w = v.ctx.h1_hi_w;
+            R.mid = v.ctx.mid;
+            R.half_w = v.ctx.half_w;
             R.h2_w = v.ctx.h2_w;
             R.tau = v.ctx.tau;
             R.cres = (v.c180s >> 8) | (v.d180s & 0xFF00u) | jj << 16;
@@ -886,12 +887,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const uint32_t qps = cj ? qn180s : qd180s, qns = cj ? qd180s : qn180s;
                     const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
                     LaneCtx qx;
-                    qx.h1_lo_w = Q.h1_lo_w;
-                    qx.h1_hi_w = Q.h1_hi_w;
+                    qx.mid = Q.mid;
+                    qx.half_w = Q.half_w;
                     qx.h2_w = Q.h2_w;
                     qx.tau = Q.tau;
-                    qx.h1_lo_n = qx.h1_lo_w + qx.tau;
-                    qx.h1_hi_n = qx.h1_hi_w - qx.tau;
+                    qx.half_n = qx.half_w - qx.tau;
                     qx.h2_n = qx.h2_w - qx.tau;
                     const uint2 ar = s_area[qh];
                     qx.inv24 = ar.x;

@@ -109,6 +109,12 @@ struct LaneCtx {
     //   term1: lo+ <= |2D| <= hi (d = +|2D|),  term2: |2D| <= -lo (d = -|2D|)
     double h1_lo_w, h1_lo_n, h1_hi_w, h1_hi_n;
     double h2_w, h2_n;
+    // term1 as one distance: |D| in [h1_lo, h1_hi] <=> ||D| - mid| <= half with
+    // mid = (lo+ + hi) / 4, half = (hi - lo+) / 4 widened (half_w) and
+    // narrowed (half_n) by tau / 2; mid, half and the distance add at most five
+    // roundings of 2^-52 N^4 <= tau / 256 each to the tau / 16 budget, still
+    // far inside the tau / 2 band
+    double mid, half_w, half_n;
     double tau;   // undecided band: FP64 errors stay below tau / 16
     double lm;    // FP64 value of l^4 + m^4
     uint32_t inv24;  // inverse of h = (3 or 9) * (1 or 5) modulo 2^24
@@ -150,6 +156,9 @@ HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4
     c.h1_lo_n = (lop + tau) * 0.5;
     c.h1_hi_w = (hi + tau) * 0.5;
     c.h1_hi_n = (hi - tau) * 0.5;
+    c.mid = (lop + hi) * 0.25;
+    c.half_w = (hi - lop) * 0.25 + tau * 0.5;
+    c.half_n = c.half_w - tau;
     c.h2_w = (-lo + tau) * 0.5;
     c.h2_n = (-lo - tau) * 0.5;
     c.tau = tau;
