@@ -27,8 +27,8 @@
 // evaluates only the left entries inside each partner's angle window (left
 // sets sorted by angle).  Leaves with counted terms are compacted with
 // ballot/mbcnt into a per-wave LDS ring and root-tested a wave of terms at a
-// time; the few survivors, and leaves left undecided by the FP64 band, go to a
-// global buffer that the exact kernel (k_exact) settles with 128/256-bit
+// time; the few survivors, and the leaves left undecided by the FP64 band, go
+// to a global buffer that the exact kernel (k_exact) settles with 128/256-bit
 // integer arithmetic.
 #include <hip/hip_runtime.h>
 
@@ -426,13 +426,13 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
     if (pred && slot < P.def_cap) P.def[slot] = it;
 }
 
-// Per-wave LDS ring of leaves for the deferred buffer: leaves with counted
-// terms awaiting the root-proximity test, and leaves the FP64 band left
-// undecided.  Entry: X, Y of z/2 and the packed id: (u | partner entry << 17,
-// k | conj << 27 | counted terms << 28), the terms as bits 0-3 for lm - 2|X|,
-// lm + 2|X|, lm - 2|Y|, lm + 2|Y|; an entry without counted terms is undecided.
-// Slots head .. head + n - 1 (mod QCAP) are live; slot QCAP takes the writes
-// of lanes without an entry.
+// Per-wave LDS ring of leaves with counted terms awaiting the root-proximity
+// test (leaves the FP64 band left undecided go straight to the deferred
+// buffer, defer_leaf).  Entry: X, Y of z/2 and the packed id: (u | partner
+// entry << 17, k | conj << 27 | counted terms << 28), the terms as bits 0-3 for
+// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear.  Slots head ..
+// head + n - 1 (mod QCAP) are live; slot QCAP takes the writes of lanes
+// without an entry.
 struct RootQueue {
     double2* xy;
     uint2* id;
@@ -447,11 +447,10 @@ struct TileCtx {
 
 __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj << ID_U_BITS; }
 
-// One round over the oldest min(n, 64) queued leaves, one per lane: an
-// undecided leaf goes to the deferred buffer for the full exact count; a leaf
-// with counted terms has its lowest remaining term root-tested (a survivor goes
-// to the deferred buffer) and is queued again while terms remain, so every
-// round tests a full wave of terms.
+// One round over the oldest min(n, 64) queued leaves, one per lane: the
+// lowest remaining term is root-tested (a survivor goes to the deferred
+// buffer) and the leaf is queued again while terms remain, so every round
+// tests a full wave of terms.
 __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
     const uint32_t count = q.n < 64u ? q.n : 64u;
@@ -464,17 +463,15 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     q.head = (q.head + count) & (uint32_t)(QCAP - 1);
     q.n -= count;
     const uint32_t k = id.y & 0x7FFFFFFu;
-    const uint32_t terms = in ? id.y >> 28 : 0u;
-    const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without a term
+    const uint32_t terms = in ? id.y >> 28 : 0u;  // every queued entry has a counted term
+    const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
     const double lm = lm_of(tc.q, k, tc.l4_d);
     const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
-    const bool hit = (terms != 0u) & root_near_integer((t & 1u) ? lm + a : lm - a);
-    const bool pass = in & ((terms == 0u) | hit);
+    const bool pass = in & root_near_integer((t & 1u) ? lm + a : lm - a);
     if (__builtin_amdgcn_ballot_w64(pass)) {
         const uint32_t ui = id.x & MAX_LEFT_ENTRIES, jj = id.x >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
-        const uint32_t kflags = k | ((id.y >> 27) & 1u) << 30 | (terms == 0u ? 0x80000000u : 0u);
-        push_global(P, pass, DefItem{tc.left, ui, vidx, kflags});
+        push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((id.y >> 27) & 1u) << 30});
     }
     const uint32_t rest = terms & (terms - 1u);
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
@@ -506,13 +503,14 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
 // for lane data, with area residues pre-shifted (vXs = vX << 8).  Adds the
-// decided count to cnt and returns the leaf's queue flags: its counted terms
-// (bits 0-3) and 16 when the FP64 band left it undecided; 0 when there is
-// nothing to queue.  ONE: the pair has h2_w < 0 (tile_one_term), so term2 of
+// decided count to cnt and returns the leaf's counted terms (bits 0-3: term1,
+// term2 of orientation 1, then of orientation 2); undecided is set (and no
+// term returned) when the FP64 band leaves the leaf to the exact stage.  ONE:
+// the pair has h2_w < 0 (tile_one_term), so term2 of
 // either orientation is false and its compares are skipped (same result).
 template <bool ONE = false>
 __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t vXs, uint32_t vYs, const LaneCtx& c,
-                                       uint32_t& cnt) {
+                                       uint32_t& cnt, bool& undecided) {
     // bitwise logic on lane masks: no short-circuit branches
     const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
     const bool a1 = live & area_ok_s(vYs, c);
@@ -540,21 +538,33 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
     const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
     cnt += (uint32_t)__builtin_popcount(terms);
-    return terms | (und ? 16u : 0u);
+    undecided = und;
+    return terms;
 }
 
-// Queue a decided leaf (flags from leaf_decide; kq = k | conj << 27, idlo =
-// pack_ui(ui, partner entry)): with counted terms for the root-proximity test,
-// undecided for the exact count.
-__device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t flags, double X,
+// Queue a decided leaf with counted terms (from leaf_decide; kq = k | conj <<
+// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test.
+__device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t terms, double X,
                                   double Y, uint32_t kq, uint32_t idlo) {
-    root_push(P, rq, tc, flags != 0u, X, Y, idlo, kq | (flags & 15u) << 28);
+    root_push(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
+}
+
+// An undecided leaf goes whole to the exact stage (k_exact counts all its
+// terms): its partner entry from (k, jj), marked as a full leaf.  Called by all
+// lanes (push_global is a wave operation), rarely with any und set.
+__device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool und, uint32_t kq, uint32_t idlo) {
+    const uint32_t k = kq & 0x7FFFFFFu;
+    const uint32_t vidx = und ? P.off[k] + (idlo >> ID_U_BITS) : 0u;
+    push_global(P, und, DefItem{tc.left, idlo & MAX_LEFT_ENTRIES, vidx, k | ((kq >> 27) & 1u) << 30 | 0x80000000u});
 }
 
 __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
                                  uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
                                  uint32_t& cnt) {
-    leaf_queue(P, rq, tc, leaf_decide(live, X, Y, vXs, vYs, c, cnt), X, Y, kq, idlo);
+    bool und;
+    const uint32_t terms = leaf_decide(live, X, Y, vXs, vYs, c, cnt, und);
+    if (__builtin_amdgcn_ballot_w64(und)) defer_leaf(P, tc, und, kq, idlo);
+    leaf_queue(P, rq, tc, terms, X, Y, kq, idlo);
 }
 
 // The per-pair context and residues of one partner entry v (lane data).
@@ -742,8 +752,13 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                     const uint32_t vX1 = pac + __umul24(u.rb, v.d180s), vY1 = pbc + __umul24(u.ra, v.n180s);
                     const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
                     const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
-                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, cnt);
-                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, cnt);
+                    bool u0, u1;
+                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, cnt, u0);
+                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, cnt, u1);
+                    if (__builtin_amdgcn_ballot_w64(u0 | u1)) {  // rare
+                        defer_leaf(P, tc, u0, kj, ui | jpk);
+                        defer_leaf(P, tc, u1, kj | 1u << 27, ui | jpk);
+                    }
                     leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
                     leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
                 }
@@ -900,20 +915,29 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const uint32_t jpk = (qr >> 16) << ID_U_BITS;
                     double UX[WIN_UNIT], UY[WIN_UNIT];
                     uint32_t UF[WIN_UNIT];
+                    bool UU[WIN_UNIT];
+                    const uint32_t nlive = uend > ui0 ? uend - ui0 : 0u;  // live entries of the unit
                     // the unit's entries from one base: ui0 <= nsort, and the left
                     // entries carry WIN_UNIT entries of padding, so every load is in
                     // bounds; entries past uend are masked by live
                     const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
 #pragma unroll
                     for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-                        const bool live = ui0 + s < uend;
+                        const bool live = s < nlive;
                         const LeftFast u = up[s];
                         const double t1 = u.im * qd, t2 = u.re * qd;
                         UX[s] = __builtin_fma(u.re, qc, -t1);
                         UY[s] = __builtin_fma(u.im, qc, t2);
                         const uint32_t vXs = __umul24(u.ra, qc180s) + __umul24(u.rb, qns);
                         const uint32_t vYs = __umul24(u.rb, qc180s) + __umul24(u.ra, qps);
-                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, cnt);
+                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, cnt, UU[s]);
+                    }
+                    bool uany = false;
+#pragma unroll
+                    for (uint32_t s = 0; s < WIN_UNIT; ++s) uany |= UU[s];
+                    if (__builtin_amdgcn_ballot_w64(uany)) {  // rare
+#pragma unroll
+                        for (uint32_t s = 0; s < WIN_UNIT; ++s) defer_leaf(P, tc, UU[s], kq, (ui0 + s) | jpk);
                     }
 #pragma unroll
                     for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk);
