@@ -430,9 +430,11 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
 // test (leaves the FP64 band left undecided go straight to the deferred
 // buffer, defer_leaf).  Entry: X, Y of z/2 and the packed id: (u | partner
 // entry << 17, k | conj << 27 | counted terms << 28), the terms as bits 0-3 for
-// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear.  Slots head ..
-// head + n - 1 (mod QCAP) are live; slot QCAP takes the writes of lanes
-// without an entry.
+// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).
+// In the ZZ form (window tiles with one term per orientation: bits 0 and 2
+// only) the entry holds lm - 2|X| and lm - 2|Y| in place of X and Y, so a
+// round needs no lm.  Slots head .. head + n - 1 (mod QCAP) are live; slot
+// QCAP takes the writes of lanes without an entry.
 struct RootQueue {
     double2* xy;
     uint2* id;
@@ -451,6 +453,7 @@ __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj <<
 // lowest remaining term is root-tested (a survivor goes to the deferred
 // buffer) and the leaf is queued again while terms remain, so every round
 // tests a full wave of terms.
+template <bool ZZ>
 __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
     const uint32_t count = q.n < 64u ? q.n : 64u;
@@ -465,9 +468,15 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     const uint32_t k = id.y & 0x7FFFFFFu;
     const uint32_t terms = in ? id.y >> 28 : 0u;  // every queued entry has a counted term
     const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
-    const double lm = lm_of(tc.q, k, tc.l4_d);
-    const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
-    const bool pass = in & root_near_integer((t & 1u) ? lm + a : lm - a);
+    double zz;
+    if constexpr (ZZ) {
+        zz = t < 2u ? xy.x : xy.y;  // t is 0 or 2
+    } else {
+        const double lm = lm_of(tc.q, k, tc.l4_d);
+        const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
+        zz = (t & 1u) ? lm + a : lm - a;
+    }
+    const bool pass = in & root_near_integer(zz);
     if (__builtin_amdgcn_ballot_w64(pass)) {
         const uint32_t ui = id.x & MAX_LEFT_ENTRIES, jj = id.x >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
@@ -486,6 +495,7 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
 
 // Append the lanes with pred set to the queue; full waves of entries are
 // drained (the queue never holds more than 127 live entries).
+template <bool ZZ>
 __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCtx& tc, bool pred, double X, double Y,
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
@@ -494,11 +504,12 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
     q.xy[slot] = make_double2(X, Y);
     q.id[slot] = make_uint2(idlo, idhi);
     q.n += (uint32_t)__popcll(mask);
-    while (q.n >= 64) root_round(P, q, tc);
+    while (q.n >= 64) root_round<ZZ>(P, q, tc);
 }
 
+template <bool ZZ>
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
-    while (q.n) root_round(P, q, tc);
+    while (q.n) root_round<ZZ>(P, q, tc);
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
@@ -543,10 +554,16 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
 }
 
 // Queue a decided leaf with counted terms (from leaf_decide; kq = k | conj <<
-// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test.
+// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test; ZZ: the
+// queue form with lm - 2|X|, lm - 2|Y| (one rounding each, as in the rounds).
+template <bool ZZ = false>
 __device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t terms, double X,
-                                  double Y, uint32_t kq, uint32_t idlo) {
-    root_push(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
+                                  double Y, uint32_t kq, uint32_t idlo, double lm = 0.0) {
+    if constexpr (ZZ)
+        root_push<true>(P, rq, tc, terms != 0u, __builtin_fma(-2.0, __builtin_fabs(X), lm),
+                        __builtin_fma(-2.0, __builtin_fabs(Y), lm), idlo, kq | terms << 28);
+    else
+        root_push<false>(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
 }
 
 // An undecided leaf goes whole to the exact stage (k_exact counts all its
@@ -558,13 +575,14 @@ __device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool u
     push_global(P, und, DefItem{tc.left, idlo & MAX_LEFT_ENTRIES, vidx, k | ((kq >> 27) & 1u) << 30 | 0x80000000u});
 }
 
+template <bool ZZ = false>
 __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
                                  uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
                                  uint32_t& cnt) {
     bool und;
     const uint32_t terms = leaf_decide(live, X, Y, vXs, vYs, c, cnt, und);
     if (__builtin_amdgcn_ballot_w64(und)) defer_leaf(P, tc, und, kq, idlo);
-    leaf_queue(P, rq, tc, terms, X, Y, kq, idlo);
+    leaf_queue<ZZ>(P, rq, tc, terms, X, Y, kq, idlo, c.lm);
 }
 
 // The per-pair context and residues of one partner entry v (lane data).
@@ -619,6 +637,7 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
 
 // The central (real) entry u of the left set against v: leaf u*v only; with a
 // real v too, the real-leaf rule.
+template <bool ZZ = false>
 __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
                                     const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
     const uint32_t ua = u.ra, ub = u.rb;
@@ -632,8 +651,8 @@ __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const Ti
     // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
     cnt += (a2 && v.ctx.h2_n >= 0.0) ? 1u : 0u;
     push_global(P, a2 && v.ctx.h2_w >= 0.0 && v.ctx.h2_n < 0.0, DefItem{tc.left, ui, v.vidx, v.kj | 0x80000000u});
-    leaf_fast(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), vXs, vYs, v.ctx,
-              v.kj, pack_ui(ui, v.jj), cnt);
+    leaf_fast<ZZ>(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), vXs, vYs,
+                  v.ctx, v.kj, pack_ui(ui, v.jj), cnt);
 }
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
@@ -769,7 +788,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 rows(std::false_type{});
             central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
         });
-        root_flush(P, rq, tc);
+        root_flush<false>(P, rq, tc);
         tile_done(P, S, cnt, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
     }
 }
@@ -792,7 +811,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 // the area-test constants from the class of h (3, 9, 15, 45).
 struct WinRec {
     double c, d;  // partner v
-    double mid, half_w, h2_w, tau;
+    double mid, half_w, h2_w, tau;  // h2_w holds l^4 + m^4 on one-term tiles
     uint32_t cres;    // c mod 180 | (d mod 180) << 8 | partner entry << 16
     uint32_t kjh;     // k | class of h << 28
     uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
@@ -839,7 +858,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
             pair_ctx(L, S, kj, v.ctx);
-            central_leaf(P, rq, tc, act, lf[nsort], nsort, v, cnt);
+            // one-term tiles queue in the ZZ form (l^4 + m^4 is v.ctx.lm)
+            if (one)
+                central_leaf<true>(P, rq, tc, act, lf[nsort], nsort, v, cnt);
+            else
+                central_leaf<false>(P, rq, tc, act, lf[nsort], nsort, v, cnt);
             // ranges of non-central entries
             const double md = (double)L.q * (double)kj;  // exact: m < 2^26
             const double m2 = md * md;                      // exact: m^2 < 2^52
@@ -868,7 +891,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             R.d = v.d;
             R.mid = v.ctx.mid;
             R.half_w = v.ctx.half_w;
-            R.h2_w = v.ctx.h2_w;
+            R.h2_w = one ? v.ctx.lm : v.ctx.h2_w;  // one-term tiles need l^4 + m^4 instead
             R.tau = v.ctx.tau;
             R.cres = (v.c180s >> 8) | (v.d180s & 0xFF00u) | jj << 16;
             R.kjh = kj | hcl << 28;
@@ -904,7 +927,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     LaneCtx qx;
                     qx.mid = Q.mid;
                     qx.half_w = Q.half_w;
-                    qx.h2_w = Q.h2_w;
+                    qx.h2_w = Q.h2_w;  // l^4 + m^4 on one-term tiles (unused by leaf_decide<true>)
                     qx.tau = Q.tau;
                     qx.half_n = qx.half_w - qx.tau;
                     qx.h2_n = qx.h2_w - qx.tau;
@@ -940,7 +963,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                         for (uint32_t s = 0; s < WIN_UNIT; ++s) defer_leaf(P, tc, UU[s], kq, (ui0 + s) | jpk);
                     }
 #pragma unroll
-                    for (uint32_t s = 0; s < WIN_UNIT; ++s) leaf_queue(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk);
+                    for (uint32_t s = 0; s < WIN_UNIT; ++s)
+                        leaf_queue<ONE>(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk, Q.h2_w);
                 }
             };
             if (one)
@@ -949,7 +973,10 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 phase2(std::false_type{});
             __builtin_amdgcn_wave_barrier();
         });
-        root_flush(P, rq, tc);
+        if (one)
+            root_flush<true>(P, rq, tc);
+        else
+            root_flush<false>(P, rq, tc);
         tile_done(P, S, cnt, leaves, pairs, evald, P.evaluated);
     }
 }
