@@ -596,9 +596,12 @@ struct Partner {
 };
 
 // The pair context of m = q * k (make_lane_ctx).
+// (make_lane_ctx only asks whether 3 and 5 divide m: the divisibility tests
+// k * 3^-1 <= (2^32 - 1) / 3 and k * 5^-1 <= (2^32 - 1) / 5 modulo 2^32.)
 __device__ inline void pair_ctx(const LeftDesc& L, const PosDesc& S, uint32_t k, LaneCtx& c) {
-    const uint32_t m3 = (L.q3 * (k % 3u)) % 3u, m5 = (L.q5 * (k % 5u)) % 5u;
-    make_lane_ctx((uint64_t)L.q * k, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, c);
+    const uint32_t m3 = (L.q3 == 0u || k * 0xAAAAAAABu <= 0x55555555u) ? 0u : 1u;
+    const uint32_t m5 = (L.q5 == 0u || k * 0xCCCCCCCDu <= 0x33333333u) ? 0u : 1u;
+    make_lane_ctx_d((double)L.q * (double)k, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, c);
 }
 
 // Per-k record of the widened bounds (the narrowed ones are w -+ tau) and the

@@ -142,10 +142,10 @@ HSS_HD inline double lm_of(uint32_t q, uint32_t k, double l4_d) {
 }
 
 // Per-pair context.  l4_d, n4_d: FP64 values of l^4 and N^4; l2_d = l^2;
-// l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5.
-HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
-                                 uint32_t m3, uint32_t m5, LaneCtx& c) {
-    const double md = (double)(uint32_t)m;  // exact: m < 2^26
+// l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5 (only whether they
+// are 0 matters); md = m as an FP64 value (exact: m < 2^26).
+HSS_HD inline void make_lane_ctx_d(double md, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
+                                   uint32_t m3, uint32_t m5, LaneCtx& c) {
     double m2 = md * md;                     // exact: m^2 < 2^52
     double m4 = m2 * m2;
     double lo = (l4_d + m4) - n4_d;
@@ -169,6 +169,11 @@ HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4
     constexpr uint32_t I3 = inv_mod_2_24(3), I9 = inv_mod_2_24(9), I15 = inv_mod_2_24(15), I45 = inv_mod_2_24(45);
     c.inv24 = need9 ? (need5 ? I45 : I9) : (need5 ? I15 : I3);
     c.thr22 = need9 ? (need5 ? 0x3FFFFFu / 45 : 0x3FFFFFu / 9) : (need5 ? 0x3FFFFFu / 15 : 0x3FFFFFu / 3);
+}
+
+HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
+                                 uint32_t m3, uint32_t m5, LaneCtx& c) {
+    make_lane_ctx_d((double)(uint32_t)m, l4_d, l2_d, n4_d, l3, l5, m3, m5, c);
 }
 
 // area4 = 2A: area4 = 0 (mod 8) and |area4| = 0 (mod h), from any va = A (mod
@@ -532,18 +537,21 @@ HSS_HD inline uint32_t merge4(uint32_t* s, uint32_t* e) {
 template <uint32_t NB>
 HSS_HD inline uint32_t window_ranges(const Window& w, const uint32_t* bt, uint32_t nsort, double psi_v, bool conj,
                                      uint32_t* s, uint32_t* e) {
+    // branch-free (lanes of a wave hold different pairs): both arcs are always
+    // mapped, then the whole circle (n < 0) and a single arc (n == 1) select
     const double shift = conj ? psi_v : -psi_v;
-    if (w.n < 0) {
-        s[0] = 0;
-        e[0] = nsort;
-        s[1] = e[1] = s[2] = e[2] = s[3] = e[3] = 0;
-        return nsort;
-    }
-    cyclic_ranges<NB>(bt, w.a0, w.b0, shift, s[0], e[0], s[1], e[1]);
-    if (w.n == 2)
-        cyclic_ranges<NB>(bt, w.a1, w.b1, shift, s[2], e[2], s[3], e[3]);
-    else
-        s[2] = e[2] = s[3] = e[3] = 0;
+    uint32_t s0, e0, s1, e1, s2, e2, s3, e3;
+    cyclic_ranges<NB>(bt, w.a0, w.b0, shift, s0, e0, s1, e1);
+    cyclic_ranges<NB>(bt, w.a1, w.b1, shift, s2, e2, s3, e3);
+    const bool all = w.n < 0, two = w.n == 2;
+    s[0] = all ? 0u : s0;
+    e[0] = all ? nsort : e0;
+    s[1] = all ? 0u : s1;
+    e[1] = all ? 0u : e1;
+    s[2] = two ? s2 : 0u;
+    e[2] = two ? e2 : 0u;
+    s[3] = two ? s3 : 0u;
+    e[3] = two ? e3 : 0u;
     return merge4(s, e);
 }
 
