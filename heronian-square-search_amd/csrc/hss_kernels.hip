@@ -452,12 +452,13 @@ __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj <<
 // One round over the oldest min(n, 64) queued leaves, one per lane: the
 // lowest remaining term is root-tested (a survivor goes to the deferred
 // buffer) and the leaf is queued again while terms remain, so every round
-// tests a full wave of terms.
-template <bool ZZ>
+// tests a full wave of terms.  FULL: the queue holds at least 64 entries
+// (every lane has one; the lane masks fold away).
+template <bool ZZ, bool FULL>
 __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
-    const uint32_t count = q.n < 64u ? q.n : 64u;
-    const bool in = lane < count;
+    const uint32_t count = FULL ? 64u : (q.n < 64u ? q.n : 64u);
+    const bool in = FULL || lane < count;
     __builtin_amdgcn_wave_barrier();
     const uint32_t at = (q.head + lane) & (uint32_t)(QCAP - 1);
     const double2 xy = q.xy[at];
@@ -504,12 +505,12 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
     q.xy[slot] = make_double2(X, Y);
     q.id[slot] = make_uint2(idlo, idhi);
     q.n += (uint32_t)__popcll(mask);
-    while (q.n >= 64) root_round<ZZ>(P, q, tc);
+    while (q.n >= 64) root_round<ZZ, true>(P, q, tc);
 }
 
 template <bool ZZ>
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
-    while (q.n) root_round<ZZ>(P, q, tc);
+    while (q.n) root_round<ZZ, false>(P, q, tc);
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
