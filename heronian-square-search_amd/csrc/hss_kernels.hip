@@ -460,7 +460,9 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     const uint32_t count = FULL ? 64u : (q.n < 64u ? q.n : 64u);
     const bool in = FULL || lane < count;
     __builtin_amdgcn_wave_barrier();
-    const uint32_t at = (q.head + lane) & (uint32_t)(QCAP - 1);
+    // full rounds keep head a multiple of 64 (the queue starts, and every
+    // flush ends, with head 0), so head + lane needs no wrap
+    const uint32_t at = FULL ? q.head + lane : (q.head + lane) & (uint32_t)(QCAP - 1);
     const double2 xy = q.xy[at];
     const uint2 id = q.id[at];
     __builtin_amdgcn_wave_barrier();
@@ -483,7 +485,8 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
         push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((id.y >> 27) & 1u) << 30});
     }
-    const uint32_t rest = terms & (terms - 1u);
+    // terms left after the lowest (ZZ: terms are 1, 4 or 5)
+    const uint32_t rest = ZZ ? (terms == 5u ? 4u : 0u) : terms & (terms - 1u);
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
     if (again) {
         const uint32_t slot = rest != 0u ? (q.head + q.n + lane_rank(again)) & (uint32_t)(QCAP - 1) : (uint32_t)QCAP;
@@ -511,6 +514,7 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
 template <bool ZZ>
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     while (q.n) root_round<ZZ, false>(P, q, tc);
+    q.head = 0;  // empty: restart at slot 0 (full rounds rely on head = 0 or 64)
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
