@@ -312,31 +312,33 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
     }
     pc.clear();
     // Window mode per left set: eligible by size, then per tile by the
-    // window fraction of its partner range.  Work is estimated in plain-leaf
-    // units: a plain cell costs 1, a window-mode cell f * win_item_cost plus
-    // win_partner_cost / H per partner entry.
+    // window fraction of its partner range.  Work is estimated in plain-cell
+    // units per partner entry: a plain tile H + plain_partner_cost (the cells
+    // plus the flattening, partner loads and central leaf of each entry), a
+    // window-mode tile H f win_item_cost + win_partner_cost.
     const uint32_t nl = (uint32_t)P.lefts.size();
     const uint32_t lchunks = std::max(1u, std::min(256u, nl / 64));
     std::vector<uint8_t> eligible(nl, 0);
-    std::vector<double> gain(nl, 1.0);  // cost per cell of the whole left set
+    std::vector<double> ecost(nl, 0.0);  // cost per partner entry of the whole left set
     std::vector<double> fmax(nl, 0.0);
     parallel_chunks(nl, lchunks, [&](uint32_t b, uint32_t e, uint32_t) {
         for (uint32_t li = b; li < e; ++li) {
             const LeftDesc& L = P.lefts[li];
             const uint32_t H = L.H;
+            const double plain = (double)H + tune.plain_partner_cost;
+            ecost[li] = plain;
             const int wc = weight_class(H);
             if (wc < 0 || !(tune.win_max_frac[wc] > 0.0)) continue;
             eligible[li] = 1;
             fmax[li] = tune.win_max_frac[wc];
             const double f =
                 tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, P.kmin[li], P.kmax[li], 9);
-            if (f <= fmax[li]) gain[li] = f * tune.win_item_cost + tune.win_partner_cost / H;
-            if (gain[li] > 1.0) gain[li] = 1.0;
+            if (f <= fmax[li]) ecost[li] = std::min(plain, (double)H * f * tune.win_item_cost + tune.win_partner_cost);
         }
     });
     double total = 0;
     for (uint32_t li = 0; li < nl; ++li)
-        total += (double)P.lefts[li].H * (double)entries(off, P.kmin[li], P.kmax[li]) * gain[li];
+        total += (double)entries(off, P.kmin[li], P.kmax[li]) * ecost[li];
     // tile size: a few tiles per resident wavefront, bounded
     const uint32_t tpw = tune.tiles_per_wave ? tune.tiles_per_wave : 8;
     double target = total / (double)(waves ? waves * tpw : tpw);
@@ -358,7 +360,7 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
         for (uint32_t li = b; li < e; ++li) {
             const LeftDesc& L = P.lefts[li];
             uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
-            uint64_t budget = (uint64_t)std::max(target / ((double)L.H * gain[li]), 256.0);
+            uint64_t budget = (uint64_t)std::max(target / ecost[li], 256.0);
             while (k0 <= kmax) {
                 // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
                 uint32_t lo = k0, hi = kmax;
@@ -371,7 +373,7 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                 }
                 uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
                 const double cells = (double)L.H * (double)entries(off, k0, kend);
-                double cost = cells;
+                double cost = cells + (double)entries(off, k0, kend) * tune.plain_partner_cost;
                 bool win = false;
                 if (eligible[li]) {
                     const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);

@@ -297,6 +297,7 @@ extern "C" int hss_ctx_get_tuning(const hss_ctx* c, hss_tuning* out) {
     for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i) out->win_max_frac[i] = c->tune.win_max_frac[i];
     out->win_item_cost = c->tune.win_item_cost;
     out->win_partner_cost = c->tune.win_partner_cost;
+    out->plain_partner_cost = c->tune.plain_partner_cost;
     return HSS_OK;
 }
 
@@ -307,11 +308,13 @@ extern "C" int hss_ctx_set_tuning(hss_ctx* c, const hss_tuning* in) {
     for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i)
         if (!(in->win_max_frac[i] >= 0.0 && in->win_max_frac[i] <= 1.0))
             return fail(HSS_EINVAL, "win_max_frac outside [0, 1]");
-    if (!(in->win_item_cost > 0.0) || !(in->win_partner_cost >= 0.0)) return fail(HSS_EINVAL, "bad cost weights");
+    if (!(in->win_item_cost > 0.0) || !(in->win_partner_cost >= 0.0) || !(in->plain_partner_cost >= 0.0))
+        return fail(HSS_EINVAL, "bad cost weights");
     c->tune.tiles_per_wave = in->tiles_per_wave;
     for (int i = 0; i < HSS_WEIGHT_CLASSES; ++i) c->tune.win_max_frac[i] = in->win_max_frac[i];
     c->tune.win_item_cost = in->win_item_cost;
     c->tune.win_partner_cost = in->win_partner_cost;
+    c->tune.plain_partner_cost = in->plain_partner_cost;
     return HSS_OK;
 }
 

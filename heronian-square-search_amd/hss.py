@@ -54,11 +54,12 @@ WEIGHT_CLASSES = 9
 class Tuning(ctypes.Structure):
     _fields_ = [("tiles_per_wave", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("win_max_frac", ctypes.c_double * WEIGHT_CLASSES), ("win_item_cost", ctypes.c_double),
-                ("win_partner_cost", ctypes.c_double)]
+                ("win_partner_cost", ctypes.c_double), ("plain_partner_cost", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {"tiles_per_wave": self.tiles_per_wave, "win_max_frac": list(self.win_max_frac),
-                "win_item_cost": self.win_item_cost, "win_partner_cost": self.win_partner_cost}
+                "win_item_cost": self.win_item_cost, "win_partner_cost": self.win_partner_cost,
+                "plain_partner_cost": self.plain_partner_cost}
 
     @classmethod
     def from_dict(cls, d: dict) -> "Tuning":
@@ -68,6 +69,7 @@ class Tuning(ctypes.Structure):
             t.win_max_frac[i] = v
         t.win_item_cost = d["win_item_cost"]
         t.win_partner_cost = d["win_partner_cost"]
+        t.plain_partner_cost = d.get("plain_partner_cost", 0.0)
         return t
 
 

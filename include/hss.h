@@ -133,6 +133,7 @@ typedef struct {
     double win_max_frac[HSS_WEIGHT_CLASSES];
     double win_item_cost;    /* planning cost of a window-mode leaf vs a plain leaf */
     double win_partner_cost; /* planning cost per partner entry of a window-mode tile */
+    double plain_partner_cost; /* planning cost per partner entry of a plain tile (cell units) */
 } hss_tuning;
 
 /*
