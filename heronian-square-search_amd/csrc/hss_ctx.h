@@ -62,11 +62,13 @@ struct Counters {
 struct LaunchBuffers {
     DevBuf header, pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
     HostBuf upload;   // pinned staging of the plan
+    HostBuf upload2;  // pinned staging of the tiles when they are uploaded after the setup stage
     HostBuf h_misc;   // pinned copy of misc: [1] deferred, [2] raw, [3] hits, [6..9] evaluated (2 x u64)
     uint32_t def_cap = 0;
     static constexpr int STAGES = 4;
-    // start, setup done, window cells done, plain cells done, exact done, misc read back
-    hipEvent_t ev[6] = {};
+    // start, setup done, window cells done, plain cells done, exact done, misc
+    // read back, cells start (after the tile upload when it follows the setup)
+    hipEvent_t ev[7] = {};
     Plan plan;
     uint32_t npos = 0;
     // The kernel chain (left-set sort and build with the counter reset; the
@@ -143,6 +145,14 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n);
 // the positions' counters, cells, exact and acceptance kernels, read-back of
 // the misc counters into pinned memory.  Asynchronous.
 int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
+// The same launch in two parts around plan_tiles: launch_plan_lefts, then
+// launch_enqueue_setup (uploads the positions and left sets, runs the left-set
+// sort and build and the counter reset), launch_plan_tiles (host, while the
+// device sorts), launch_enqueue_cells (tiles, cell kernels, exact stage).
+int launch_plan_lefts(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n);
+int launch_plan_tiles(hss_ctx* c, LaunchBuffers& B);
+int launch_enqueue_setup(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
+int launch_enqueue_cells(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st);
 // After B.ev[5] has completed: read the misc counters, detect overflow, copy
 // accepted hits on `copy` (a stream with no work that writes B's buffers).
 int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out);

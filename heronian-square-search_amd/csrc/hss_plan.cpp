@@ -206,8 +206,8 @@ double tile_window_fraction(uint64_t l, uint64_t limit, uint64_t q, uint32_t k0,
     return sum / samples;
 }
 
-bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t>& small_primes, const uint32_t* off,
-                uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& P, std::string& err) {
+bool plan_lefts(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                uint64_t n_max, const Tuning& tune, Plan& P, std::string& err) {
     P = Plan();
     P.pos.resize(n_roots);
     // positions: factorisation, signatures and left-set recipes, in chunks
@@ -318,9 +318,12 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
     // window-mode tile H f win_item_cost + win_partner_cost.
     const uint32_t nl = (uint32_t)P.lefts.size();
     const uint32_t lchunks = std::max(1u, std::min(256u, nl / 64));
-    std::vector<uint8_t> eligible(nl, 0);
-    std::vector<double> ecost(nl, 0.0);  // cost per partner entry of the whole left set
-    std::vector<double> fmax(nl, 0.0);
+    std::vector<uint8_t>& eligible = P.eligible;
+    std::vector<double>& ecost = P.ecost;  // cost per partner entry of the whole left set
+    std::vector<double>& fmax = P.fmax;
+    eligible.assign(nl, 0);
+    ecost.assign(nl, 0.0);
+    fmax.assign(nl, 0.0);
     parallel_chunks(nl, lchunks, [&](uint32_t b, uint32_t e, uint32_t) {
         for (uint32_t li = b; li < e; ++li) {
             const LeftDesc& L = P.lefts[li];
@@ -336,6 +339,28 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
             if (f <= fmax[li]) ecost[li] = std::min(plain, (double)H * f * tune.win_item_cost + tune.win_partner_cost);
         }
     });
+    // every left set that window mode may use is stored sorted (its plain
+    // tiles read the entries in that order just as well), so the sort can
+    // start before the tiles are planned
+    for (uint32_t li = 0; li < nl; ++li) {
+        P.lefts[li].bt = NO_TABLE;
+        if (eligible[li]) {
+            P.lefts[li].bt = (uint32_t)P.sorted.size();
+            P.sorted.push_back(li);
+        }
+    }
+    return true;
+}
+
+bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P) {
+    const uint32_t nl = (uint32_t)P.lefts.size();
+    const uint32_t lchunks = std::max(1u, std::min(256u, nl / 64));
+    const std::vector<uint8_t>& eligible = P.eligible;
+    const std::vector<double>& ecost = P.ecost;
+    const std::vector<double>& fmax = P.fmax;
+    P.tiles.clear();
+    P.wtiles.clear();
+    P.est_work = P.win_cells = 0;
     double total = 0;
     for (uint32_t li = 0; li < nl; ++li)
         total += (double)entries(off, P.kmin[li], P.kmax[li]) * ecost[li];
@@ -354,7 +379,6 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
         double win_cells = 0, work = 0;
     };
     std::vector<TileChunk> tc(lchunks);
-    std::vector<uint8_t> sort_me(nl, 0);
     parallel_chunks(nl, lchunks, [&](uint32_t b, uint32_t e, uint32_t c) {
         TileChunk& C = tc[c];
         for (uint32_t li = b; li < e; ++li) {
@@ -381,7 +405,6 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                         win = true;
                         cost = cells * f * tune.win_item_cost +
                                (double)entries(off, k0, kend) * tune.win_partner_cost;
-                        sort_me[li] = 1;
                         C.win_cells += cells;
                     }
                 }
@@ -405,13 +428,6 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
         P.est_work += C.work;
     }
     tc.clear();
-    for (uint32_t li = 0; li < nl; ++li) {
-        P.lefts[li].bt = NO_TABLE;
-        if (sort_me[li]) {
-            P.lefts[li].bt = (uint32_t)P.sorted.size();
-            P.sorted.push_back(li);
-        }
-    }
     // heaviest tiles first, per mode: a counting sort on the cost quantised to
     // 1/8 octaves (load balance needs the order only roughly; linear time)
     constexpr int KEYS = 8 * 64;
@@ -430,6 +446,11 @@ bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
     P.tiles.reserve(order.size() - nwin);
     for (uint32_t i : sorted) (order[i].win ? P.wtiles : P.tiles).push_back(all[order[i].i]);
     return true;
+}
+
+bool plan_batch(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& P, std::string& err) {
+    return plan_lefts(reqs, n_roots, small_primes, off, n_max, tune, P, err) && plan_tiles(off, waves, tune, P);
 }
 
 }  // namespace hss

@@ -59,6 +59,10 @@ struct Plan {
     double est_cells = 0;   // upper bound of (left entry, partner entry) cells
     double est_work = 0;    // estimated work after window restriction (plain-leaf units / 2)
     double win_cells = 0;   // cells of the window-mode tiles
+    // per left set, from plan_lefts for plan_tiles: window mode possible,
+    // its window-fraction threshold, estimated cost per partner entry
+    std::vector<uint8_t> eligible;
+    std::vector<double> fmax, ecost;
 };
 
 // primes up to and including `upto`
@@ -80,6 +84,13 @@ struct PosReq {
 // unit of work.
 bool plan_batch(const PosReq* reqs, uint32_t n, const std::vector<uint32_t>& small_primes, const uint32_t* off,
                 uint64_t n_max, uint64_t waves, const Tuning& tune, Plan& out, std::string& err);
+
+// The two halves of plan_batch: positions, left sets and their window
+// eligibility (every eligible left set is listed in `sorted`), then the tiles.
+// The left-set build and sort may run on the device while plan_tiles runs.
+bool plan_lefts(const PosReq* reqs, uint32_t n, const std::vector<uint32_t>& small_primes, const uint32_t* off,
+                uint64_t n_max, const Tuning& tune, Plan& P, std::string& err);
+bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P);
 
 // Mean range-window fraction over partners m = q*k, k in [k0, k1], of root l at
 // limit (sampled at a few k).

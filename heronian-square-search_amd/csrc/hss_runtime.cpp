@@ -88,6 +88,7 @@ void LaunchBuffers::release() {
                       &def, &raw, &hits})
         b->release();
     upload.release();
+    upload2.release();
     h_misc.release();
     for (hipEvent_t& e : ev) {
         if (e) (void)hipEventDestroy(e);
@@ -425,6 +426,19 @@ int launch_plan(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
     return HSS_OK;
 }
 
+int launch_plan_lefts(hss_ctx* c, LaunchBuffers& B, const PosReq* reqs, uint32_t n) {
+    std::string err;
+    if (!plan_lefts(reqs, n, c->small_primes, c->h_off.data(), c->nmax, c->tune, B.plan, err))
+        return fail(HSS_EINVAL, err);
+    B.npos = n;
+    return HSS_OK;
+}
+
+int launch_plan_tiles(hss_ctx* c, LaunchBuffers& B) {
+    if (!plan_tiles(c->h_off.data(), c->waves(), c->tune, B.plan)) return fail(HSS_EINVAL, "tile planning failed");
+    return HSS_OK;
+}
+
 // The kernel chain of one launch in four stages on stream st (timing events
 // go between the stages): 0 = left-set sort and build and the counter reset,
 // 1 = window-mode cell kernel, 2 = plain cell kernel, 3 = exact and
@@ -486,17 +500,15 @@ static bool capture_stages(hss_ctx* c, LaunchBuffers& B, const CellParams& K) {
     return true;
 }
 
-int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
+// Device buffers of the positions and left sets (setup) and of the tiles.
+static int reserve_setup(LaunchBuffers& B) {
     const Plan& P = B.plan;
-    const uint32_t npos = B.npos, nleft = (uint32_t)P.lefts.size(), ntiles = (uint32_t)P.tiles.size();
-    const uint32_t nwtiles = (uint32_t)P.wtiles.size(), nsorted = (uint32_t)P.sorted.size();
+    const uint32_t nleft = (uint32_t)P.lefts.size(), nsorted = (uint32_t)P.sorted.size();
     const size_t entries = std::max<uint64_t>(P.left_entries, 1);
     HIP_TRY(B.header.reserve(sizeof(BatchHeader)));
-    HIP_TRY(B.pos.reserve(std::max<size_t>(npos, 1) * sizeof(PosDesc)));
+    HIP_TRY(B.pos.reserve(std::max<size_t>(B.npos, 1) * sizeof(PosDesc)));
     HIP_TRY(B.lefts.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftDesc)));
     HIP_TRY(B.builds.reserve(std::max<size_t>(nleft, 1) * sizeof(LeftBuild)));
-    HIP_TRY(B.tiles.reserve(std::max<size_t>(ntiles, 1) * sizeof(TileDesc)));
-    HIP_TRY(B.wtiles.reserve(std::max<size_t>(nwtiles, 1) * sizeof(TileDesc)));
     HIP_TRY(B.sorted.reserve(std::max<size_t>(nsorted, 1) * 4));
     HIP_TRY(B.btab.reserve(std::max<size_t>(nsorted, 1) * (WIN_NB + 1) * 4));
     HIP_TRY(B.perm.reserve(entries * 4));
@@ -510,46 +522,81 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
         B.def_cap = 1u << 22;
         HIP_TRY(B.def.reserve((size_t)B.def_cap * sizeof(DefItem)));
     }
-    // stage the header and the plan in pinned memory: one region per array, 256-byte aligned
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t b_hdr = al(sizeof(BatchHeader)), b_pos = al(npos * sizeof(PosDesc)),
-                 b_left = al(nleft * sizeof(LeftDesc)), b_build = al(nleft * sizeof(LeftBuild)),
-                 b_tile = al(ntiles * sizeof(TileDesc)), b_wtile = al(nwtiles * sizeof(TileDesc)),
-                 b_sorted = al(nsorted * 4);
-    HIP_TRY(B.upload.reserve(b_hdr + b_pos + b_left + b_build + b_tile + b_wtile + b_sorted));
-    char* const up_hdr = B.upload.as<char>();
+    return HSS_OK;
+}
+
+static int reserve_cells(LaunchBuffers& B) {
+    HIP_TRY(B.tiles.reserve(std::max<size_t>(B.plan.tiles.size(), 1) * sizeof(TileDesc)));
+    HIP_TRY(B.wtiles.reserve(std::max<size_t>(B.plan.wtiles.size(), 1) * sizeof(TileDesc)));
+    return HSS_OK;
+}
+
+static BatchHeader batch_header(const LaunchBuffers& B, bool with_tiles) {
+    BatchHeader hdr{};
+    hdr.npos = B.npos;
+    hdr.nleft = (uint32_t)B.plan.lefts.size();
+    hdr.nsorted = (uint32_t)B.plan.sorted.size();
+    hdr.ntiles = with_tiles ? (uint32_t)B.plan.tiles.size() : 0u;
+    hdr.nwtiles = with_tiles ? (uint32_t)B.plan.wtiles.size() : 0u;
+    return hdr;
+}
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Stage the header (with or without tile counts), positions, left sets and
+// sorted list in `up` and copy them to the device.
+static int upload_setup(LaunchBuffers& B, HostBuf& up, bool with_tiles, hipStream_t st) {
+    const Plan& P = B.plan;
+    const uint32_t npos = B.npos, nleft = (uint32_t)P.lefts.size(), nsorted = (uint32_t)P.sorted.size();
+    const size_t b_hdr = al256(sizeof(BatchHeader)), b_pos = al256(npos * sizeof(PosDesc)),
+                 b_left = al256(nleft * sizeof(LeftDesc)), b_build = al256(nleft * sizeof(LeftBuild)),
+                 b_sorted = al256(nsorted * 4);
+    HIP_TRY(up.reserve(b_hdr + b_pos + b_left + b_build + b_sorted));
+    char* const up_hdr = up.as<char>();
     char* const up_pos = up_hdr + b_hdr;
     char* const up_left = up_pos + b_pos;
     char* const up_build = up_left + b_left;
-    char* const up_tile = up_build + b_build;
-    char* const up_wtile = up_tile + b_tile;
-    char* const up_sorted = up_wtile + b_wtile;
-    BatchHeader hdr{};
-    hdr.npos = npos;
-    hdr.nleft = nleft;
-    hdr.nsorted = nsorted;
-    hdr.ntiles = ntiles;
-    hdr.nwtiles = nwtiles;
+    char* const up_sorted = up_build + b_build;
+    const BatchHeader hdr = batch_header(B, with_tiles);
     std::memcpy(up_hdr, &hdr, sizeof hdr);
     std::memcpy(up_pos, P.pos.data(), npos * sizeof(PosDesc));
     std::memcpy(up_left, P.lefts.data(), nleft * sizeof(LeftDesc));
     std::memcpy(up_build, P.builds.data(), nleft * sizeof(LeftBuild));
-    std::memcpy(up_tile, P.tiles.data(), ntiles * sizeof(TileDesc));
-    std::memcpy(up_wtile, P.wtiles.data(), nwtiles * sizeof(TileDesc));
     std::memcpy(up_sorted, P.sorted.data(), nsorted * 4);
-
-    HIP_TRY(hipEventRecord(B.ev[0], st));
     HIP_TRY(hipMemcpyAsync(B.header.p, up_hdr, sizeof hdr, hipMemcpyHostToDevice, st));
     if (npos) HIP_TRY(hipMemcpyAsync(B.pos.p, up_pos, npos * sizeof(PosDesc), hipMemcpyHostToDevice, st));
     if (nleft) {
         HIP_TRY(hipMemcpyAsync(B.lefts.p, up_left, nleft * sizeof(LeftDesc), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(B.builds.p, up_build, nleft * sizeof(LeftBuild), hipMemcpyHostToDevice, st));
     }
+    if (nsorted) HIP_TRY(hipMemcpyAsync(B.sorted.p, up_sorted, nsorted * 4, hipMemcpyHostToDevice, st));
+    return HSS_OK;
+}
+
+// Stage the tiles (and, with_header, the full header) in `up` and copy them.
+static int upload_cells(LaunchBuffers& B, HostBuf& up, bool with_header, hipStream_t st) {
+    const Plan& P = B.plan;
+    const uint32_t ntiles = (uint32_t)P.tiles.size(), nwtiles = (uint32_t)P.wtiles.size();
+    const size_t b_hdr = al256(sizeof(BatchHeader)), b_tile = al256(ntiles * sizeof(TileDesc)),
+                 b_wtile = al256(nwtiles * sizeof(TileDesc));
+    HIP_TRY(up.reserve(b_hdr + b_tile + b_wtile));
+    char* const up_hdr = up.as<char>();
+    char* const up_tile = up_hdr + b_hdr;
+    char* const up_wtile = up_tile + b_tile;
+    std::memcpy(up_tile, P.tiles.data(), ntiles * sizeof(TileDesc));
+    std::memcpy(up_wtile, P.wtiles.data(), nwtiles * sizeof(TileDesc));
+    if (with_header) {
+        const BatchHeader hdr = batch_header(B, true);
+        std::memcpy(up_hdr, &hdr, sizeof hdr);
+        HIP_TRY(hipMemcpyAsync(B.header.p, up_hdr, sizeof hdr, hipMemcpyHostToDevice, st));
+    }
     if (ntiles) HIP_TRY(hipMemcpyAsync(B.tiles.p, up_tile, ntiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
     if (nwtiles)
         HIP_TRY(hipMemcpyAsync(B.wtiles.p, up_wtile, nwtiles * sizeof(TileDesc), hipMemcpyHostToDevice, st));
-    if (nsorted) HIP_TRY(hipMemcpyAsync(B.sorted.p, up_sorted, nsorted * 4, hipMemcpyHostToDevice, st));
+    return HSS_OK;
+}
 
+static CellParams cell_params(hss_ctx* c, LaunchBuffers& B, const Counters& cnt) {
     // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile, [6..9] evaluated
     uint32_t* misc = B.misc.as<uint32_t>();
     CellParams K{};
@@ -579,7 +626,14 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
     K.nraw = misc + 2;
     K.raw_cap = LaunchBuffers::RAW_CAP;
     K.qr = c->qr;
+    return K;
+}
 
+// Stages first .. STAGES - 1, replayed from graphs when they are available
+// (captured once per buffer layout), then the read-back of misc.
+static int run_stages(hss_ctx* c, LaunchBuffers& B, const CellParams& K, const Counters& cnt, int first,
+                      hipStream_t st) {
+    const uint32_t ntiles = (uint32_t)B.plan.tiles.size(), nwtiles = (uint32_t)B.plan.wtiles.size();
     bool replay = false;
     if (c->use_graphs) {
         std::vector<uintptr_t> layout;
@@ -597,7 +651,7 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
         }
         replay = B.exec[0] != nullptr;
     }
-    for (int stage = 0; stage < LaunchBuffers::STAGES; ++stage) {
+    for (int stage = first; stage < LaunchBuffers::STAGES; ++stage) {
         const bool empty = (stage == 1 && !nwtiles) || (stage == 2 && !ntiles);  // nothing to run
         if (!empty && replay) HIP_TRY(hipGraphLaunch(B.exec[stage], st));
         if (!empty && !replay) {
@@ -605,12 +659,44 @@ int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_
             if (rc != HSS_OK) return rc;
         }
         HIP_TRY(hipEventRecord(B.ev[1 + stage], st));
+        if (stage == 0) HIP_TRY(hipEventRecord(B.ev[6], st));
     }
     B.replayed = replay;
     vlog(st, "kernel chain");
-    HIP_TRY(hipMemcpyAsync(B.h_misc.p, misc, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 48, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(B.ev[5], st));
     return HSS_OK;
+}
+
+int launch_enqueue(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
+    int rc = reserve_setup(B);
+    if (rc == HSS_OK) rc = reserve_cells(B);
+    if (rc != HSS_OK) return rc;
+    HIP_TRY(hipEventRecord(B.ev[0], st));
+    rc = upload_setup(B, B.upload, true, st);
+    if (rc == HSS_OK) rc = upload_cells(B, B.upload2, false, st);
+    if (rc != HSS_OK) return rc;
+    return run_stages(c, B, cell_params(c, B, cnt), cnt, 0, st);
+}
+
+int launch_enqueue_setup(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
+    int rc = reserve_setup(B);
+    if (rc != HSS_OK) return rc;
+    HIP_TRY(hipEventRecord(B.ev[0], st));
+    rc = upload_setup(B, B.upload, false, st);  // no tiles yet: their counts follow with them
+    if (rc != HSS_OK) return rc;
+    rc = enqueue_stage(c, B, cell_params(c, B, cnt), 0, st);
+    if (rc != HSS_OK) return rc;
+    HIP_TRY(hipEventRecord(B.ev[1], st));
+    return HSS_OK;
+}
+
+int launch_enqueue_cells(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipStream_t st) {
+    int rc = reserve_cells(B);
+    if (rc == HSS_OK) rc = upload_cells(B, B.upload2, true, st);
+    if (rc != HSS_OK) return rc;
+    HIP_TRY(hipEventRecord(B.ev[6], st));  // the window kernel's time starts here, not at the end of the setup
+    return run_stages(c, B, cell_params(c, B, cnt), cnt, 1, st);
 }
 
 int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out) {
@@ -623,7 +709,7 @@ int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult&
     out.evaluated = out.win_evaluated + out.plain_evaluated;
     out.graph = B.replayed;
     (void)hipEventElapsedTime(&out.setup_ms, B.ev[0], B.ev[1]);
-    (void)hipEventElapsedTime(&out.win_ms, B.ev[1], B.ev[2]);
+    (void)hipEventElapsedTime(&out.win_ms, B.ev[6], B.ev[2]);
     (void)hipEventElapsedTime(&out.plain_ms, B.ev[2], B.ev[3]);
     (void)hipEventElapsedTime(&out.exact_ms, B.ev[3], B.ev[4]);
     out.cells_ms = out.win_ms + out.plain_ms;
@@ -664,17 +750,22 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     std::vector<PosReq> reqs(n_roots);
     for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limit, s};
     LaunchBuffers& B = c->sync;
-    int rc = launch_plan(c, B, reqs.data(), n_roots);
-    if (rc != HSS_OK) return rc;
-    c->stats.plan_ms = ms_since(t0);
     HIP_TRY(c->counters.reserve((size_t)n_roots * 3 * 8));
     Counters cnt;
     cnt.cand = c->counters.as<unsigned long long>();
     cnt.leaves = cnt.cand + n_roots;
     cnt.pairs = cnt.cand + 2 * (size_t)n_roots;
+    // left sets first: their sort and build run on the device while the host
+    // plans the tiles
+    int rc = launch_plan_lefts(c, B, reqs.data(), n_roots);
+    if (rc == HSS_OK) rc = launch_enqueue_setup(c, B, cnt, c->stream);
+    if (rc == HSS_OK) rc = launch_plan_tiles(c, B);
+    if (rc != HSS_OK) return rc;
+    c->stats.plan_ms = ms_since(t0);
     LaunchResult R;
     for (int attempt = 0;; ++attempt) {
-        rc = launch_enqueue(c, B, cnt, c->stream);
+        // a second attempt (deferred buffer grown) replays the whole chain
+        rc = attempt == 0 ? launch_enqueue_cells(c, B, cnt, c->stream) : launch_enqueue(c, B, cnt, c->stream);
         if (rc != HSS_OK) return rc;
         HIP_TRY(hipEventSynchronize(B.ev[5]));
         rc = launch_collect(c, B, c->stream, R);
