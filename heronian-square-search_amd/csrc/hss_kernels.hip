@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
 
 static constexpr int WG = 256;
 static constexpr int WAVES = WG / 64;
-static constexpr int QCAP = 128;  // root queue ring (a power of two), plus one dummy slot
+static constexpr int QCAP = 128;  // root queue ring (a power of two)
 
 // Inclusive prefix sum / maximum over the wavefront with DPP: shifts by 1, 2,
 // 4, 8 within each row of 16 lanes, then the last lane of rows 0 and 2 into
@@ -406,8 +406,8 @@ __device__ inline uint64_t wave_sum64(uint64_t v) {
     return v;
 }
 
-__device__ inline uint32_t lane_rank(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+__device__ inline uint32_t lane_rank(uint64_t mask, uint32_t base = 0u) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, base));
 }
 
 __device__ inline bool divides(uint32_t k, uint32_t inv, uint32_t thr) { return k * inv <= thr; }
@@ -433,11 +433,15 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
 // lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).
 // In the ZZ form (window tiles with one term per orientation: bits 0 and 2
 // only) the entry holds lm - 2|X| and lm - 2|Y| in place of X and Y, so a
-// round needs no lm.  Slots head .. head + n - 1 (mod QCAP) are live; slot
-// QCAP takes the writes of lanes without an entry.
+// round needs no lm.  Slots head .. head + n - 1 (mod QCAP) are live.  One
+// 24-byte entry per slot: one address per push or pop.
+struct QEntry {
+    double x, y;
+    uint32_t lo, hi;
+};
+
 struct RootQueue {
-    double2* xy;
-    uint2* id;
+    QEntry* e;
     uint32_t head, n;
 };
 
@@ -463,35 +467,34 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     // full rounds keep head a multiple of 64 (the queue starts, and every
     // flush ends, with head 0), so head + lane needs no wrap
     const uint32_t at = FULL ? q.head + lane : (q.head + lane) & (uint32_t)(QCAP - 1);
-    const double2 xy = q.xy[at];
-    const uint2 id = q.id[at];
+    const QEntry qe = q.e[at];
     __builtin_amdgcn_wave_barrier();
     q.head = (q.head + count) & (uint32_t)(QCAP - 1);
     q.n -= count;
-    const uint32_t k = id.y & 0x7FFFFFFu;
-    const uint32_t terms = in ? id.y >> 28 : 0u;  // every queued entry has a counted term
-    const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
+    const uint32_t k = qe.hi & 0x7FFFFFFu;
+    const uint32_t terms = in ? qe.hi >> 28 : 0u;  // every queued entry has a counted term
     double zz;
     if constexpr (ZZ) {
-        zz = t < 2u ? xy.x : xy.y;  // t is 0 or 2
+        zz = (terms & 1u) ? qe.x : qe.y;  // terms are 1, 4 or 5: the lowest first
     } else {
+        const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
         const double lm = lm_of(tc.q, k, tc.l4_d);
-        const double a = 2.0 * __builtin_fabs(t < 2u ? xy.x : xy.y);
+        const double a = 2.0 * __builtin_fabs(t < 2u ? qe.x : qe.y);
         zz = (t & 1u) ? lm + a : lm - a;
     }
     const bool pass = in & root_near_integer(zz);
     if (__builtin_amdgcn_ballot_w64(pass)) {
-        const uint32_t ui = id.x & MAX_LEFT_ENTRIES, jj = id.x >> ID_U_BITS;
+        const uint32_t ui = qe.lo & MAX_LEFT_ENTRIES, jj = qe.lo >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
-        push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((id.y >> 27) & 1u) << 30});
+        push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((qe.hi >> 27) & 1u) << 30});
     }
     // terms left after the lowest (ZZ: terms are 1, 4 or 5)
     const uint32_t rest = ZZ ? (terms == 5u ? 4u : 0u) : terms & (terms - 1u);
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
     if (again) {
-        const uint32_t slot = rest != 0u ? (q.head + q.n + lane_rank(again)) & (uint32_t)(QCAP - 1) : (uint32_t)QCAP;
-        q.xy[slot] = xy;
-        q.id[slot] = make_uint2(id.x, (id.y & 0x0FFFFFFFu) | rest << 28);
+        if (rest != 0u)
+            q.e[lane_rank(again, q.head + q.n) & (uint32_t)(QCAP - 1)] =
+                QEntry{qe.x, qe.y, qe.lo, (qe.hi & 0x0FFFFFFFu) | rest << 28};
         __builtin_amdgcn_wave_barrier();
         q.n += (uint32_t)__popcll(again);
     }
@@ -504,9 +507,7 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
-    const uint32_t slot = pred ? (q.head + q.n + lane_rank(mask)) & (uint32_t)(QCAP - 1) : (uint32_t)QCAP;
-    q.xy[slot] = make_double2(X, Y);
-    q.id[slot] = make_uint2(idlo, idhi);
+    if (pred) q.e[lane_rank(mask, q.head + q.n) & (uint32_t)(QCAP - 1)] = QEntry{X, Y, idlo, idhi};
     q.n += (uint32_t)__popcll(mask);
     while (q.n >= 64) root_round<ZZ, true>(P, q, tc);
 }
@@ -732,12 +733,11 @@ __device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t
         const LeftFast* __restrict__ lfast
 
 __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
-    __shared__ double2 s_qxy[WAVES][QCAP + 1];
-    __shared__ uint2 s_qid[WAVES][QCAP + 1];
+    __shared__ QEntry s_q[WAVES][QCAP];
     __shared__ uint32_t s_mark[WAVES][65];
     __shared__ KCtx s_kctx[WAVES][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
+    RootQueue rq{s_q[wave], 0, 0};
     uint32_t* mark = s_mark[wave];
     KCtx* kctx = s_kctx[wave];
     for (;;) {
@@ -832,8 +832,7 @@ static constexpr uint32_t AREA_INV[4] = {inv_mod_2_24(3), inv_mod_2_24(9), inv_m
 static constexpr uint32_t AREA_THR[4] = {0x3FFFFFu / 3, 0x3FFFFFu / 9, 0x3FFFFFu / 15, 0x3FFFFFu / 45};
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
-    __shared__ double2 s_qxy[WAVES][QCAP + 1];
-    __shared__ uint2 s_qid[WAVES][QCAP + 1];
+    __shared__ QEntry s_q[WAVES][QCAP];
     __shared__ WinRec s_rec[WAVES][64];
     __shared__ uint32_t s_mark[WAVES][65];
     __shared__ uint2 s_area[4];  // (inv24, thr22) per class of h
@@ -844,7 +843,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                                i == 0 ? AREA_THR[0] : i == 1 ? AREA_THR[1] : i == 2 ? AREA_THR[2] : AREA_THR[3]);
     }
     __syncthreads();
-    RootQueue rq{s_qxy[wave], s_qid[wave], 0, 0};
+    RootQueue rq{s_q[wave], 0, 0};
     WinRec* rec = s_rec[wave];
     uint32_t* mark = s_mark[wave];
     for (;;) {
@@ -943,7 +942,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     qx.inv24 = ar.x;
                     qx.thr22 = ar.y;
                     const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
-                    const uint32_t jpk = (qr >> 16) << ID_U_BITS;
+                    // ids pack_ui(ui0 + s, partner entry): ui0 + s stays below 2^ID_U_BITS
+                    const uint32_t ub = ui0 | (qr >> 16) << ID_U_BITS;
                     double UX[WIN_UNIT], UY[WIN_UNIT];
                     uint32_t UF[WIN_UNIT];
                     bool UU[WIN_UNIT];
@@ -968,11 +968,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     for (uint32_t s = 0; s < WIN_UNIT; ++s) uany |= UU[s];
                     if (__builtin_amdgcn_ballot_w64(uany)) {  // rare
 #pragma unroll
-                        for (uint32_t s = 0; s < WIN_UNIT; ++s) defer_leaf(P, tc, UU[s], kq, (ui0 + s) | jpk);
+                        for (uint32_t s = 0; s < WIN_UNIT; ++s) defer_leaf(P, tc, UU[s], kq, ub + s);
                     }
 #pragma unroll
                     for (uint32_t s = 0; s < WIN_UNIT; ++s)
-                        leaf_queue<ONE>(P, rq, tc, UF[s], UX[s], UY[s], kq, (ui0 + s) | jpk, Q.h2_w);
+                        leaf_queue<ONE>(P, rq, tc, UF[s], UX[s], UY[s], kq, ub + s, Q.h2_w);
                 }
             };
             if (one)
