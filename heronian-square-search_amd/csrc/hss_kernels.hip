@@ -412,6 +412,14 @@ __device__ inline uint32_t lane_rank(uint64_t mask, uint32_t base = 0u) {
 
 __device__ inline bool divides(uint32_t k, uint32_t inv, uint32_t thr) { return k * inv <= thr; }
 
+// Element i of an LDS array of T through a 32-bit byte offset (one 24-bit
+// multiply-add): plain indexing with a stride that is not a power of two
+// leaves 64-bit address arithmetic in the emitted code.
+template <typename T>
+__device__ inline T& lds_at(T* base, uint32_t i) {
+    return *(T*)((char*)base + __umul24(i, (uint32_t)sizeof(T)));
+}
+
 // Append the lanes with pred set to the global deferred buffer.
 __device__ inline void push_global(const CellParams& P, bool pred, const DefItem& it) {
     uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
@@ -432,8 +440,9 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
 // entry << 17, k | conj << 27 | counted terms << 28), the terms as bits 0-3 for
 // lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).
 // In the ZZ form (window tiles with one term per orientation: bits 0 and 2
-// only) the entry holds lm - 2|X| and lm - 2|Y| in place of X and Y, so a
-// round needs no lm.  Slots head .. head + n - 1 (mod QCAP) are live.  One
+// only) the entry holds the zz of its lowest term (lm - 2|X| with bit 0, else
+// lm - 2|Y|) in place of X and lm - 2|Y| in place of Y, so a round needs no
+// lm and no select.  Slots head .. head + n - 1 (mod QCAP) are live.  One
 // 24-byte entry per slot: one address per push or pop.
 struct QEntry {
     double x, y;
@@ -443,6 +452,10 @@ struct QEntry {
 struct RootQueue {
     QEntry* e;
     uint32_t head, n;
+    // terms root-tested so far: every counted term of a queued leaf is tested
+    // exactly once, so this is their candidate count (a scalar add per round
+    // instead of a count per leaf)
+    uint64_t tested;
 };
 
 struct TileCtx {
@@ -466,16 +479,18 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     __builtin_amdgcn_wave_barrier();
     // full rounds keep head a multiple of 64 (the queue starts, and every
     // flush ends, with head 0), so head + lane needs no wrap
-    const uint32_t at = FULL ? q.head + lane : (q.head + lane) & (uint32_t)(QCAP - 1);
-    const QEntry qe = q.e[at];
+    // (full rounds: head is 0 or 64, so the lane's slot address plus 0 or 64 entries)
+    const QEntry qe = FULL ? *(const QEntry*)((const char*)&lds_at(q.e, lane) + (q.head ? 64u * sizeof(QEntry) : 0u))
+                           : lds_at(q.e, (q.head + lane) & (uint32_t)(QCAP - 1));
     __builtin_amdgcn_wave_barrier();
     q.head = (q.head + count) & (uint32_t)(QCAP - 1);
     q.n -= count;
+    q.tested += count;
     const uint32_t k = qe.hi & 0x7FFFFFFu;
     const uint32_t terms = in ? qe.hi >> 28 : 0u;  // every queued entry has a counted term
     double zz;
     if constexpr (ZZ) {
-        zz = (terms & 1u) ? qe.x : qe.y;  // terms are 1, 4 or 5: the lowest first
+        zz = qe.x;  // terms are 1, 4 or 5
     } else {
         const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
         const double lm = lm_of(tc.q, k, tc.l4_d);
@@ -493,8 +508,8 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
     if (again) {
         if (rest != 0u)
-            q.e[lane_rank(again, q.head + q.n) & (uint32_t)(QCAP - 1)] =
-                QEntry{qe.x, qe.y, qe.lo, (qe.hi & 0x0FFFFFFFu) | rest << 28};
+            lds_at(q.e, lane_rank(again, q.head + q.n) & (uint32_t)(QCAP - 1)) =
+                QEntry{ZZ ? qe.y : qe.x, qe.y, qe.lo, (qe.hi & 0x0FFFFFFFu) | rest << 28};
         __builtin_amdgcn_wave_barrier();
         q.n += (uint32_t)__popcll(again);
     }
@@ -507,7 +522,7 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
-    if (pred) q.e[lane_rank(mask, q.head + q.n) & (uint32_t)(QCAP - 1)] = QEntry{X, Y, idlo, idhi};
+    if (pred) lds_at(q.e, lane_rank(mask, q.head + q.n) & (uint32_t)(QCAP - 1)) = QEntry{X, Y, idlo, idhi};
     q.n += (uint32_t)__popcll(mask);
     while (q.n >= 64) root_round<ZZ, true>(P, q, tc);
 }
@@ -519,15 +534,16 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
-// for lane data, with area residues pre-shifted (vXs = vX << 8).  Adds the
-// decided count to cnt and returns the leaf's counted terms (bits 0-3: term1,
-// term2 of orientation 1, then of orientation 2); undecided is set (and no
-// term returned) when the FP64 band leaves the leaf to the exact stage.  ONE:
+// for lane data, with area residues pre-shifted (vXs = vX << 8).  Returns the
+// leaf's counted terms (bits 0-3: term1, term2 of orientation 1, then of
+// orientation 2), which are its decided candidates (counted as the root queue
+// tests them); undecided is set (and no term returned) when the FP64 band
+// leaves the leaf to the exact stage.  ONE:
 // the pair has h2_w < 0 (tile_one_term), so term2 of
 // either orientation is false and its compares are skipped (same result).
 template <bool ONE = false>
 __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t vXs, uint32_t vYs, const LaneCtx& c,
-                                       uint32_t& cnt, bool& undecided) {
+                                       bool& undecided) {
     // bitwise logic on lane masks: no short-circuit branches
     const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
     const bool a1 = live & area_ok_s(vYs, c);
@@ -554,20 +570,22 @@ __device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t v
     const bool o2 = a2 & !eq & ok;
     const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
     const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
-    cnt += (uint32_t)__builtin_popcount(terms);
     undecided = und;
     return terms;
 }
 
 // Queue a decided leaf with counted terms (from leaf_decide; kq = k | conj <<
 // 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test; ZZ: the
-// queue form with lm - 2|X|, lm - 2|Y| (one rounding each, as in the rounds).
+// queue form with the lowest term's zz and lm - 2|Y| (one rounding each, as in
+// the rounds).
 template <bool ZZ = false>
 __device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t terms, double X,
                                   double Y, uint32_t kq, uint32_t idlo, double lm = 0.0) {
-    if constexpr (ZZ)
-        root_push<true>(P, rq, tc, terms != 0u, __builtin_fma(-2.0, __builtin_fabs(X), lm),
-                        __builtin_fma(-2.0, __builtin_fabs(Y), lm), idlo, kq | terms << 28);
+    if constexpr (ZZ) {
+        const double zy = __builtin_fma(-2.0, __builtin_fabs(Y), lm);
+        root_push<true>(P, rq, tc, terms != 0u, (terms & 1u) ? __builtin_fma(-2.0, __builtin_fabs(X), lm) : zy, zy,
+                        idlo, kq | terms << 28);
+    }
     else
         root_push<false>(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
 }
@@ -583,10 +601,9 @@ __device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool u
 
 template <bool ZZ = false>
 __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
-                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo,
-                                 uint32_t& cnt) {
+                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo) {
     bool und;
-    const uint32_t terms = leaf_decide(live, X, Y, vXs, vYs, c, cnt, und);
+    const uint32_t terms = leaf_decide(live, X, Y, vXs, vYs, c, und);
     if (__builtin_amdgcn_ballot_w64(und)) defer_leaf(P, tc, und, kq, idlo);
     leaf_queue<ZZ>(P, rq, tc, terms, X, Y, kq, idlo, c.lm);
 }
@@ -645,7 +662,7 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
 }
 
 // The central (real) entry u of the left set against v: leaf u*v only; with a
-// real v too, the real-leaf rule.
+// real v too, the real-leaf rule (its count goes to cnt, not through the queue).
 template <bool ZZ = false>
 __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
                                     const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
@@ -661,7 +678,7 @@ __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const Ti
     cnt += (a2 && v.ctx.h2_n >= 0.0) ? 1u : 0u;
     push_global(P, a2 && v.ctx.h2_w >= 0.0 && v.ctx.h2_n < 0.0, DefItem{tc.left, ui, v.vidx, v.kj | 0x80000000u});
     leaf_fast<ZZ>(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), vXs, vYs,
-                  v.ctx, v.kj, pack_ui(ui, v.jj), cnt);
+                  v.ctx, v.kj, pack_ui(ui, v.jj));
 }
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
@@ -710,10 +727,11 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
 }
 
 // Tile bookkeeping shared by both cell kernels: per-position counters.
-__device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t cnt, uint64_t leaves, uint64_t pairs,
-                                 uint64_t evald, unsigned long long* evaluated) {
+// cand = the lanes' direct counts cnt plus the queue's tested terms.
+__device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t cnt, uint64_t tested,
+                                 uint64_t leaves, uint64_t pairs, uint64_t evald, unsigned long long* evaluated) {
     const uint32_t lane = __lane_id();
-    const uint64_t cand = wave_sum64((uint64_t)cnt);
+    const uint64_t cand = wave_sum64((uint64_t)cnt) + tested;
     leaves = wave_sum64(leaves);
     pairs = wave_sum64(pairs);
     evald = wave_sum64(evald);
@@ -737,7 +755,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
     __shared__ uint32_t s_mark[WAVES][65];
     __shared__ KCtx s_kctx[WAVES][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    RootQueue rq{s_q[wave], 0, 0};
+    RootQueue rq{s_q[wave], 0, 0, 0};
     uint32_t* mark = s_mark[wave];
     KCtx* kctx = s_kctx[wave];
     for (;;) {
@@ -757,14 +775,14 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             LaneCtx c;
             pair_ctx(L, S, k, c);
             __builtin_amdgcn_wave_barrier();
-            kctx[lane] = KCtx{c.mid, c.half_w, c.h2_w, c.tau, c.inv24, c.thr22};
+            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, c.inv24, c.thr22};
             __builtin_amdgcn_wave_barrier();
         };
         for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
                            [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j, uint32_t oj) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
-            ctx_from_k(kctx[j], v.ctx);
+            ctx_from_k(lds_at(kctx, j), v.ctx);
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
             // (instantiated for tiles with and without second-sign terms)
@@ -780,8 +798,8 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                     const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
                     const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
                     bool u0, u1;
-                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, cnt, u0);
-                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, cnt, u1);
+                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, u0);
+                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, u1);
                     if (__builtin_amdgcn_ballot_w64(u0 | u1)) {  // rare
                         defer_leaf(P, tc, u0, kj, ui | jpk);
                         defer_leaf(P, tc, u1, kj | 1u << 27, ui | jpk);
@@ -797,7 +815,8 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
         });
         root_flush<false>(P, rq, tc);
-        tile_done(P, S, cnt, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
+        tile_done(P, S, cnt, rq.tested, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
+        rq.tested = 0;
     }
 }
 
@@ -843,7 +862,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                                i == 0 ? AREA_THR[0] : i == 1 ? AREA_THR[1] : i == 2 ? AREA_THR[2] : AREA_THR[3]);
     }
     __syncthreads();
-    RootQueue rq{s_q[wave], 0, 0};
+    RootQueue rq{s_q[wave], 0, 0, 0};
     WinRec* rec = s_rec[wave];
     uint32_t* mark = s_mark[wave];
     for (;;) {
@@ -961,7 +980,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                         UY[s] = __builtin_fma(u.im, qc, t2);
                         const uint32_t vXs = __umul24(u.ra, qc180s) + __umul24(u.rb, qns);
                         const uint32_t vYs = __umul24(u.rb, qc180s) + __umul24(u.ra, qps);
-                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, cnt, UU[s]);
+                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, UU[s]);
                     }
                     bool uany = false;
 #pragma unroll
@@ -985,7 +1004,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             root_flush<true>(P, rq, tc);
         else
             root_flush<false>(P, rq, tc);
-        tile_done(P, S, cnt, leaves, pairs, evald, P.evaluated);
+        tile_done(P, S, cnt, rq.tested, leaves, pairs, evald, P.evaluated);
+        rq.tested = 0;
     }
 }
 
