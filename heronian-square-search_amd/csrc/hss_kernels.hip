@@ -257,8 +257,8 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.ra = smod128(z.re, QA);
-            f.rb = smod128(z.im, QA);
+            f.ra8 = smod128(z.re, QA) << 8;
+            f.rb8 = smod128(z.im, QA) << 8;
             lfast[e] = f;
         }
     }
@@ -534,7 +534,8 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 }
 
 // One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
-// for lane data, with area residues pre-shifted (vXs = vX << 8).  Returns the
+// for lane data, with the area residues of X and Y given as the area test's
+// products xX, xY (area_x).  Returns the
 // leaf's counted terms (bits 0-3: term1, term2 of orientation 1, then of
 // orientation 2), which are its decided candidates (counted as the root queue
 // tests them); undecided is set (and no term returned) when the FP64 band
@@ -542,12 +543,12 @@ __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileC
 // the pair has h2_w < 0 (tile_one_term), so term2 of
 // either orientation is false and its compares are skipped (same result).
 template <bool ONE = false>
-__device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t vXs, uint32_t vYs, const LaneCtx& c,
+__device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t xX, uint32_t xY, const LaneCtx& c,
                                        bool& undecided) {
     // bitwise logic on lane masks: no short-circuit branches
     const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
-    const bool a1 = live & area_ok_s(vYs, c);
-    const bool a2 = live & area_ok_s(vXs, c);
+    const bool a1 = live & area_ok_x(xY, c);
+    const bool a2 = live & area_ok_x(xX, c);
     // orientation 1 (d = X) and 2 (d = Y): narrowed ranges decide, the widened
     // ones minus the narrowed ones are the undecided band
     const double dx = __builtin_fabs(ax - c.mid), dy = __builtin_fabs(ay - c.mid);
@@ -601,9 +602,9 @@ __device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool u
 
 template <bool ZZ = false>
 __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
-                                 uint32_t vXs, uint32_t vYs, const LaneCtx& c, uint32_t kq, uint32_t idlo) {
+                                 uint32_t xX, uint32_t xY, const LaneCtx& c, uint32_t kq, uint32_t idlo) {
     bool und;
-    const uint32_t terms = leaf_decide(live, X, Y, vXs, vYs, c, und);
+    const uint32_t terms = leaf_decide(live, X, Y, xX, xY, c, und);
     if (__builtin_amdgcn_ballot_w64(und)) defer_leaf(P, tc, und, kq, idlo);
     leaf_queue<ZZ>(P, rq, tc, terms, X, Y, kq, idlo, c.lm);
 }
@@ -611,8 +612,8 @@ __device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileC
 // The per-pair context and residues of one partner entry v (lane data).
 struct Partner {
     double c, d;
-    uint32_t c180s, d180s;  // (c mod 180) << 8, (d mod 180) << 8
-    uint32_t n180s;         // (-d mod 180) << 8
+    uint32_t c180, d180, n180;  // c, d and -d mod 180
+    uint32_t cF, dF, nF;        // their area factors for the pair's class (area_x)
     uint32_t vidx, kj, jj;
     bool vc;                // v is the central (real) entry of its half-set
     LaneCtx ctx;
@@ -656,9 +657,16 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
     v.c = P.cre[v.vidx];
     v.d = P.cim[v.vidx];
     const uint32_t vr = P.cres[v.vidx];
-    v.c180s = (vr & 0xFFFFu) << 8;
-    v.d180s = (vr >> 8) & 0xFFFF00u;
-    v.n180s = v.d180s ? (QA << 8) - v.d180s : 0u;
+    v.c180 = vr & 0xFFFFu;
+    v.d180 = vr >> 16;
+    v.n180 = v.d180 ? QA - v.d180 : 0u;
+}
+
+// The area factors of v's residues, once v.ctx holds the pair's class.
+__device__ inline void partner_factors(Partner& v) {
+    v.cF = mul_u24(v.c180, v.ctx.inv24);
+    v.dF = mul_u24(v.d180, v.ctx.inv24);
+    v.nF = mul_u24(v.n180, v.ctx.inv24);
 }
 
 // The central (real) entry u of the left set against v: leaf u*v only; with a
@@ -666,18 +674,17 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
 template <bool ZZ = false>
 __device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
                                     const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
-    const uint32_t ua = u.ra, ub = u.rb;
     const double t1 = u.im * v.d, t2 = u.re * v.d;
-    // residues of X = re c - im d and Y = im c + re d with -d as its residue:
-    // sums of two products below 2 * 180^2 < 2^16, no subtraction
-    const uint32_t vXs = __umul24(ua, v.c180s) + __umul24(ub, v.n180s);
-    const uint32_t vYs = __umul24(ua, v.d180s) + __umul24(ub, v.c180s);
+    // area products of X = re c - im d and Y = im c + re d, with -d as its
+    // residue: residue sums below 2 * 180^2 < 2^16, no subtraction
+    const uint32_t xX = area_x(u.ra8, v.cF, u.rb8, v.nF);
+    const uint32_t xY = area_x(u.ra8, v.dF, u.rb8, v.cF);
     const bool real = act && v.vc;
-    const bool a2 = real && area_ok_s(vXs, v.ctx);
+    const bool a2 = real && area_ok_x(xX, v.ctx);
     // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
     cnt += (a2 && v.ctx.h2_n >= 0.0) ? 1u : 0u;
     push_global(P, a2 && v.ctx.h2_w >= 0.0 && v.ctx.h2_n < 0.0, DefItem{tc.left, ui, v.vidx, v.kj | 0x80000000u});
-    leaf_fast<ZZ>(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), vXs, vYs,
+    leaf_fast<ZZ>(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), xX, xY,
                   v.ctx, v.kj, pack_ui(ui, v.jj));
 }
 
@@ -783,6 +790,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
             ctx_from_k(lds_at(kctx, j), v.ctx);
+            partner_factors(v);
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
             // (instantiated for tiles with and without second-sign terms)
@@ -791,10 +799,10 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
                     const LeftFast u = lf[ui];
                     const double t1 = u.im * v.d, t2 = u.re * v.d;
-                    // residues (central_leaf): u*v has d, u*conj(v) has -d
-                    const uint32_t pac = __umul24(u.ra, v.c180s), pbc = __umul24(u.rb, v.c180s);
-                    const uint32_t vX0 = pac + __umul24(u.rb, v.n180s), vY0 = pbc + __umul24(u.ra, v.d180s);
-                    const uint32_t vX1 = pac + __umul24(u.rb, v.d180s), vY1 = pbc + __umul24(u.ra, v.n180s);
+                    // area products (central_leaf): u*v has d, u*conj(v) has -d
+                    const uint32_t pac = mul_u24(u.ra8, v.cF), pbc = mul_u24(u.rb8, v.cF);
+                    const uint32_t vX0 = pac + mul_u24(u.rb8, v.nF), vY0 = pbc + mul_u24(u.ra8, v.dF);
+                    const uint32_t vX1 = pac + mul_u24(u.rb8, v.dF), vY1 = pbc + mul_u24(u.ra8, v.nF);
                     const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
                     const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
                     bool u0, u1;
@@ -884,6 +892,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
             pair_ctx(L, S, kj, v.ctx);
+            partner_factors(v);
             // one-term tiles queue in the ZZ form (l^4 + m^4 is v.ctx.lm)
             if (one)
                 central_leaf<true>(P, rq, tc, act, lf[nsort], nsort, v, cnt);
@@ -919,7 +928,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             R.half_w = v.ctx.half_w;
             R.h2_w = one ? v.ctx.lm : v.ctx.h2_w;  // one-term tiles need l^4 + m^4 instead
             R.tau = v.ctx.tau;
-            R.cres = (v.c180s >> 8) | (v.d180s & 0xFF00u) | jj << 16;
+            R.cres = v.c180 | v.d180 << 8 | jj << 16;
             R.kjh = kj | hcl << 28;
             const uint32_t ucl = wave_incl_scan(units);
             const uint32_t uex = ucl - units;  // first unit of this lane's partner
@@ -946,9 +955,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const double qc = Q.c, qd = cj ? -Q.d : Q.d;
                     const uint32_t qr = Q.cres;
                     // residues of the unit's c and d (-d for u*conj(v)) and of minus that d
-                    const uint32_t qc180s = (qr & 0xFFu) << 8, qd180s = qr & 0xFF00u;
-                    const uint32_t qn180s = qd180s ? (QA << 8) - qd180s : 0u;
-                    const uint32_t qps = cj ? qn180s : qd180s, qns = cj ? qd180s : qn180s;
+                    const uint32_t qc180 = qr & 0xFFu, qd180 = (qr >> 8) & 0xFFu;
+                    const uint32_t qn180 = qd180 ? QA - qd180 : 0u;
+                    const uint32_t qp = cj ? qn180 : qd180, qn = cj ? qd180 : qn180;
                     const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
                     LaneCtx qx;
                     qx.mid = Q.mid;
@@ -960,6 +969,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                     const uint2 ar = s_area[qh];
                     qx.inv24 = ar.x;
                     qx.thr22 = ar.y;
+                    // their area factors (area_x)
+                    const uint32_t cF = mul_u24(qc180, ar.x), pF = mul_u24(qp, ar.x), nF = mul_u24(qn, ar.x);
                     const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
                     // ids pack_ui(ui0 + s, partner entry): ui0 + s stays below 2^ID_U_BITS
                     const uint32_t ub = ui0 | (qr >> 16) << ID_U_BITS;
@@ -978,9 +989,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                         const double t1 = u.im * qd, t2 = u.re * qd;
                         UX[s] = __builtin_fma(u.re, qc, -t1);
                         UY[s] = __builtin_fma(u.im, qc, t2);
-                        const uint32_t vXs = __umul24(u.ra, qc180s) + __umul24(u.rb, qns);
-                        const uint32_t vYs = __umul24(u.rb, qc180s) + __umul24(u.ra, qps);
-                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], vXs, vYs, qx, UU[s]);
+                        const uint32_t xX = area_x(u.ra8, cF, u.rb8, nF);
+                        const uint32_t xY = area_x(u.rb8, cF, u.ra8, pF);
+                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], xX, xY, qx, UU[s]);
                     }
                     bool uany = false;
 #pragma unroll

@@ -182,6 +182,8 @@ HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4
 // (2^22 - 1) / h (divisibility by an odd h through its inverse, the factor 4
 // through the rotated-out low bits).  vs * inv24 mod 2^32 = x << 8, so the
 // rotation is a 32-bit rotation by 10 of that product.
+HSS_HD inline bool area_ok_x(uint32_t x, const LaneCtx& c) { return ((x >> 10) | (x << 22)) <= c.thr22; }
+
 HSS_HD inline bool area_ok_s(uint32_t vs, const LaneCtx& c) {
 #if defined(__HIP_DEVICE_COMPILE__)
     uint32_t x;  // both operands are below 2^24: one 24-bit multiply, no masking
@@ -189,7 +191,26 @@ HSS_HD inline bool area_ok_s(uint32_t vs, const LaneCtx& c) {
 #else
     const uint32_t x = vs * c.inv24;
 #endif
-    return ((x >> 10) | (x << 22)) <= c.thr22;
+    return area_ok_x(x, c);
+}
+
+// The low 32 bits of the product of the low 24 bits of a and b (v_mul_u32_u24).
+HSS_HD inline uint32_t mul_u24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return (uint32_t)((uint64_t)(a & 0xFFFFFFu) * (uint64_t)(b & 0xFFFFFFu));
+#endif
+}
+
+// The area test's x without its multiply by h^-1: with residues r (mod 180)
+// stored as r8 = r << 8 and a partner residue r' stored as its area factor
+// F = r' * inv24 (any representative of r' h^-1 mod 2^24),
+// mul_u24(r8, F) = (r r' h^-1 mod 2^24) << 8 (mod 2^32), and the sum of two
+// such products is x = va * h^-1 << 8 (mod 2^32) for va = r1 r1' + r2 r2'
+// (< 2^16), exactly what area_ok_s forms from va << 8.
+HSS_HD inline uint32_t area_x(uint32_t r8a, uint32_t Fa, uint32_t r8b, uint32_t Fb) {
+    return mul_u24(r8a, Fa) + mul_u24(r8b, Fb);
 }
 
 HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) { return area_ok_s(va << 8, c); }

@@ -76,7 +76,7 @@ struct TileDesc {
 // FP64 view of a left entry used by the hot loop (one scalar load).
 struct LeftFast {
     double re, im;  // FP64 values (exact below 2^53, else correctly rounded)
-    uint32_t ra, rb;  // re mod 180, im mod 180
+    uint32_t ra8, rb8;  // (re mod 180) << 8, (im mod 180) << 8
 };
 
 // Sizes of one batch, read by the kernels from device memory so that the

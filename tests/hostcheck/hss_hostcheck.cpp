@@ -185,8 +185,8 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.ra = smod128(z.re, QA);
-            f.rb = smod128(z.im, QA);
+            f.ra8 = smod128(z.re, QA) << 8;
+            f.rb8 = smod128(z.im, QA) << 8;
             lfast[L.off + t] = f;
         }
     }
@@ -204,7 +204,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         const LeftFast& u = lfast[L.off + ui];
         const double c = T.cre[vidx], d = T.cim[vidx];
         const uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
-        const uint32_t ua = u.ra, ub = u.rb;
+        const uint32_t ua = u.ra8 >> 8, ub = u.rb8 >> 8;
         const double t1 = u.im * d, t2 = u.re * d;
         const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
         ++S.cells;
@@ -333,6 +333,43 @@ uint64_t hc_area_mismatches(void) {
                     const uint32_t h = ((l3 == 0 || m3 == 0) ? 9u : 3u) * ((l5 != 0 && m5 != 0) ? 5u : 1u);
                     for (uint32_t va = 0; va < 65536; ++va) bad += area_ok(va, c) != (va % (4 * h) == 0);
                 }
+    return bad;
+}
+
+// The device's area products against the area test's definition: for every
+// class h, every single product mul_u24(r << 8, r' * h^-1) must equal
+// (r r' h^-1 mod 2^24) << 8, and `samples` random residue quadruples must give
+// area_ok_x(area_x(...)) == area_ok_s((r1 r1' + r2 r2') << 8) == (4h | va).
+uint64_t hc_area_x_mismatches(uint64_t samples, uint64_t seed) {
+    uint64_t bad = 0;
+    static const uint32_t hs[4] = {3, 9, 15, 45};
+    for (uint32_t hi = 0; hi < 4; ++hi) {
+        const uint32_t h = hs[hi];
+        LaneCtx c;
+        // a context of this class (make_lane_ctx picks h from the residues of l and m mod 3, 5)
+        make_lane_ctx(7, 1.0, 1.0, 2.0, h % 9 == 0 ? 0u : 1u, h % 5 == 0 ? 1u : 0u, h % 9 == 0 ? 0u : 1u,
+                      h % 5 == 0 ? 1u : 0u, c);
+        if (mul_u24(h, c.inv24) % (1u << 24) != 1u) ++bad;  // the class's inverse
+        for (uint32_t r = 0; r < QA; ++r)
+            for (uint32_t rp = 0; rp < QA; ++rp) {
+                const uint32_t want = (uint32_t)(((uint64_t)r * rp * c.inv24) % (1ull << 24)) << 8;
+                bad += mul_u24(r << 8, mul_u24(rp, c.inv24)) != want;
+            }
+        uint64_t x = seed + hi;
+        auto next = [&] {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            return (uint32_t)(x % QA);
+        };
+        for (uint64_t t = 0; t < samples; ++t) {
+            const uint32_t r1 = next(), p1 = next(), r2 = next(), p2 = next();
+            const uint32_t va = r1 * p1 + r2 * p2;
+            const bool a = area_ok_x(area_x(r1 << 8, mul_u24(p1, c.inv24), r2 << 8, mul_u24(p2, c.inv24)), c);
+            bad += a != area_ok_s(va << 8, c);
+            bad += a != (va % (4 * h) == 0);
+        }
+    }
     return bad;
 }
 

@@ -248,6 +248,16 @@ def test_area_test_is_divisibility(hc):
     assert hc.hc_area_mismatches() == 0
 
 
+def test_area_products(hc):
+    """The kernels fold the multiply by h^-1 into the residue products (area
+    factors r' h^-1 per partner, left residues stored << 8): every single
+    product against (r r' h^-1 mod 2^24) << 8, and sums of two against the
+    plain area test and against divisibility by 4h."""
+    hc.hc_area_x_mismatches.restype = ctypes.c_uint64
+    hc.hc_area_x_mismatches.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    assert hc.hc_area_x_mismatches(200000, 20261203) == 0
+
+
 def test_device_root_candidate(hc):
     """The device fourth-root test: two reciprocal square roots, one Newton step
     (root4_refine), then the single candidate n = rint(r) checked by
