@@ -9,9 +9,17 @@
 // "HIT l m n area A" and "PROGRESS resume R completed_l D candidates C seconds S
 // pairs/s X" stdout lines, and the coverage skip of HERON_COVERAGE_FILE
 // (default coverage-latest.json, src/search_v18.cpp:254-283): positions inside
-// the report's complete_intervals count as done without being searched.
-// THREADS is accepted for compatibility; the work runs on HIP device
-// $HSS_DEVICE (default 0).  Positions are searched in batches of 512.
+// the report's complete_intervals count as done without being searched.  As in
+// the reference (src/search_v18.cpp:213-222) the report is read again every
+// 10 seconds (between batches; covered intervals only accumulate) and a
+// PROGRESS line is printed every 10 seconds and at the end; STATE is rewritten
+// after every batch.  THREADS is accepted for compatibility; the work runs on
+// HIP device $HSS_DEVICE (default 0).  Positions are searched in batches of at
+// most 512 and at most BATCH_CELLS planner cells (hss_estimate), so a SIGTERM
+// (src/search_v18.cpp:330-335) ends the run within about a second of device
+// work.  HSS_SEARCH_REFRESH_S and HSS_SEARCH_PROGRESS_S override the two
+// 10-second periods (tests use 0: every batch).
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <iostream>
@@ -66,6 +74,18 @@ std::vector<std::pair<uint64_t, uint64_t>> read_coverage(uint64_t n) {
     return got;
 }
 
+// a period in seconds from the environment (default when unset or invalid)
+double period_from_env(const char* name, double dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    char* end = nullptr;
+    const double x = std::strtod(v, &end);
+    return (end && *end == 0 && x >= 0) ? x : dflt;
+}
+
+// planner cells per batch: about a second of device work at LIMIT 1e7
+constexpr double BATCH_CELLS = 5e11;
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -114,32 +134,62 @@ int main(int argc, char** argv) {
               << " seconds\n";
     { std::ofstream touch(results_file, std::ios::app); }
     auto seen = cli::read_seen(results_file);
-    auto covered = read_coverage(n);
+    // covered root intervals; a refresh adds the report's intervals, never removes
+    std::vector<std::pair<uint64_t, uint64_t>> covered;
+    auto refresh = [&]() {
+        for (auto& iv : read_coverage(n))
+            if (std::find(covered.begin(), covered.end(), iv) == covered.end()) covered.push_back(iv);
+    };
+    refresh();
     auto is_covered = [&](uint64_t l) {
         for (auto& iv : covered)
             if (iv.first <= l && l <= iv.second) return true;
         return false;
     };
+    const double refresh_s = period_from_env("HSS_SEARCH_REFRESH_S", 10.0);
+    const double progress_s = period_from_env("HSS_SEARCH_PROGRESS_S", 10.0);
     const uint64_t begin = first;
     uint64_t next = first, pairs = 0, candidates = 0, completed = 0;
     auto start = std::chrono::steady_clock::now();
+    auto last_refresh = start, last_progress = start;
+    auto since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    };
     auto save = [&]() {
         cli::write_state(state_file, "version 1\nlimit " + std::to_string(n) + "\nbegin " + std::to_string(begin) +
                                          "\nend " + std::to_string(last) + "\nresume " + std::to_string(next) +
                                          "\npairs " + std::to_string(pairs) + "\ncandidates " +
                                          std::to_string(candidates) + "\n");
-        double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    };
+    auto progress = [&]() {
+        double s = since(start);
         std::cout << "PROGRESS resume " << next << " completed_l " << completed << " candidates " << candidates
                   << " seconds " << (uint64_t)s << " pairs/s " << (uint64_t)(pairs / std::max(s, 1.0))
                   << std::endl;
+        last_progress = std::chrono::steady_clock::now();
     };
     std::vector<hss_hit> hits(4096);
     std::vector<uint64_t> roots;
     while (next <= last && !cli::stop_requested.load()) {
-        uint64_t batch = std::min<uint64_t>(512, last - next + 1);
+        if (since(last_refresh) >= refresh_s) {
+            refresh();
+            last_refresh = std::chrono::steady_clock::now();
+        }
+        // up to 512 positions and BATCH_CELLS cells of uncovered positions (at least one)
+        uint64_t batch = 0;
+        double cells = 0;
         roots.clear();
-        for (uint64_t l = next; l < next + batch; ++l)
-            if (!is_covered(l)) roots.push_back(l);
+        while (batch < 512 && next + batch <= last) {
+            const uint64_t l = next + batch;
+            if (!is_covered(l)) {
+                double e = 0;
+                if (hss_estimate(ctx, n, l, l, &e) != HSS_OK) e = 0;
+                if (!roots.empty() && cells + e > BATCH_CELLS) break;
+                cells += e;
+                roots.push_back(l);
+            }
+            ++batch;
+        }
         uint64_t c = 0;
         uint32_t nh = 0;
         if (!roots.empty()) {
@@ -166,8 +216,10 @@ int main(int argc, char** argv) {
         completed += batch;
         next += batch;
         save();
+        if (since(last_progress) >= progress_s) progress();
     }
-    if (completed == 0) save();
+    save();
+    progress();
     hss_ctx_destroy(ctx);
     return 0;
 }

@@ -75,6 +75,42 @@ def cases():
         add(f"l1e7_heavy_{l}", 10_000_000, l, l)
     for l in light[16::32]:
         add(f"l1e7_light_{l}", 10_000_000, l, l)
+    # third wave: every root of the uneven1536 batch, one position per case
+    for l in range(7_500_000, 7_500_512):
+        add(f"l1e7_slice_{l}", 10_000_000, l, l)
+    for l in heavy:
+        add(f"l1e7_heavy_{l}", 10_000_000, l, l)
+    for l in light:
+        add(f"l1e7_light_{l}", 10_000_000, l, l)
+    # the served stream: every LIMIT-10^5 request whole, seeded positions of the others
+    for name, limit, first, last in stream_cases(STREAM_REQUESTS):
+        add(name, limit, first, last)
+    return out
+
+
+STREAM_REQUESTS = 200
+
+
+def stream_cases(n):
+    """Fixture cases for the first n requests of workloads.served_stream.
+
+    A LIMIT-10^5 request is recorded whole (one case over its interval).  A
+    LIMIT-10^6 request contributes two positions and a LIMIT-10^7 request one,
+    drawn with a generator seeded from workloads.SEED, so the sample does not
+    depend on which requests a run serves.  Names carry the request index.
+    """
+    rng = np.random.default_rng(workloads.SEED + 2)
+    out = []
+    for i, (_, limit, first, last) in enumerate(workloads.served_stream(n)):
+        picks = rng.integers(first, last + 1, 2)
+        if limit == 100_000:
+            out.append((f"stream{i:03d}_{limit}_{first}_{last}", limit, first, last))
+        elif limit == 1_000_000:
+            for l in sorted({int(x) for x in picks}):
+                out.append((f"stream{i:03d}_{limit}_pos_{l}", limit, l, l))
+        else:
+            l = int(picks[0])
+            out.append((f"stream{i:03d}_{limit}_pos_{l}", limit, l, l))
     return out
 
 
@@ -111,7 +147,11 @@ def main():
     have = set()
     if OUT.exists():
         have = {json.loads(ln)["name"] for ln in OUT.read_text().splitlines() if ln.strip()}
-    todo = [c for c in cases() if c["name"] not in have and c["name"].startswith(args.only)]
+    todo = []
+    for c in cases():
+        if c["name"] not in have and c["name"].startswith(args.only):
+            have.add(c["name"])
+            todo.append(c)
     # cheap cases first so partial runs are still useful
     todo.sort(key=lambda c: (c["limit"], c["last"] - c["first"]))
     print(f"{len(todo)} cases to run", flush=True)

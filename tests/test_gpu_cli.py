@@ -164,6 +164,50 @@ def test_search_coverage_skip(tmp_path):
     assert result_lines((tmp_path / "r").read_text()) == []
 
 
+def test_search_coverage_refresh_mid_run(tmp_path):
+    """The coverage report is read again while the run goes on
+    (src/search_v18.cpp:213-222, 254-283): a report that appears mid-run stops
+    the search of the intervals it covers.  The run is paused (SIGSTOP) after
+    its first batch, the report written, and the run continued; with the
+    refresh period 0 the next batch sees it."""
+    import hss
+    limit, first, last = 1_000_000, 1, 12_000
+    cov, state, results = tmp_path / "cov.json", tmp_path / "s.txt", tmp_path / "r.txt"
+    env = dict(os.environ, HERON_COVERAGE_FILE=str(cov), HSS_SEARCH_REFRESH_S="0", HSS_SEARCH_PROGRESS_S="0")
+    p = subprocess.Popen([str(CPU), str(limit), str(first), str(last), "1", str(state), str(results)], cwd=tmp_path,
+                         env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    try:
+        line = p.stdout.readline()
+        assert line.startswith("PROGRESS"), line
+        p.send_signal(signal.SIGSTOP)
+        resume = read_state(state)["resume"]  # the batch in flight starts here
+        cut = resume + 2048
+        assert cut < last
+        cov.write_text(json.dumps({"limit": limit, "complete_intervals": [[cut, last]], "root_coverage": {}}))
+        p.send_signal(signal.SIGCONT)
+        out, _ = p.communicate(timeout=600)
+    finally:
+        if p.poll() is None:
+            p.kill()
+    assert p.returncode == 0
+    st = read_state(state)
+    assert st["resume"] == last + 1 and st["pairs"] == sum(limit - l + 1 for l in range(first, last + 1))
+    with hss.Context(limit) as ctx:
+        assert st["candidates"] == ctx.search(limit, range(first, cut)).total
+    # PROGRESS after every batch with period 0, ending at last + 1
+    resumes = [int(ln.split()[2]) for ln in (line + out).splitlines() if ln.startswith("PROGRESS")]
+    assert resumes[-1] == last + 1 and all(a <= b for a, b in zip(resumes, resumes[1:]))
+
+
+def test_search_progress_cadence(tmp_path):
+    """With the default 10-second period a short run prints one PROGRESS line,
+    at its end (src/search_v18.cpp:213-226)."""
+    r = run([CPU, 100000, 1, 3000, 1, tmp_path / "s.txt", tmp_path / "r.txt"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("PROGRESS")]
+    assert len(lines) == 1 and lines[0].split()[:5] == ["PROGRESS", "resume", "3001", "completed_l", "3000"]
+
+
 def weight(x: int) -> int:
     """Gaussian weight by trial division (experiments/heavy/README.md:9)."""
     r, p = 1, 2
@@ -216,6 +260,24 @@ def test_heavy_weight_mode(tmp_path):
         want = ctx.search(limit, roots)
     assert st["candidates"] == want.total
     assert result_lines(results.read_text()) == want.hits
+
+
+def test_heavy_weight_reference_batch(tmp_path):
+    """The reference's own heavy-weight run: gpu_search_v36 on [1116384, 1120479]
+    at LIMIT 1e7 with min_weight 100 (experiments/heavy/heavy_queue.py:12-14)
+    searched 166 rows and counted 1,105,278,882,993 candidates
+    (evidence/benchmarks/heavy-benchmark.json:10-23)."""
+    limit, first, last, w = 10_000_000, 1116384, 1120479, 100
+    stem = f"heavy-v36-w{w}-{first}-{last}"
+    state, results = tmp_path / f"{stem}-state.txt", tmp_path / f"{stem}-results.txt"
+    r = run([GPU, limit, first, last, results, 0, state, w], tmp_path)
+    assert r.returncode == 0, r.stderr
+    st = read_state(state)
+    assert st == {"version": 1, "limit": limit, "begin": first, "end": last, "resume": last + 1,
+                  "candidates": 1_105_278_882_993, "min_weight": w}
+    roots = check_done(state, first, last + 1, limit, w)
+    assert len(roots) == 166
+    assert results.read_text() == ""  # new_solutions 0
 
 
 def test_heavy_weight_resume_and_mismatch(tmp_path):
