@@ -131,6 +131,11 @@ struct hss_ctx {
     std::string graph_error;           // why capture was given up, if it was
     hipStream_t capture = nullptr;     // stream the chain is captured on (never executes work)
     hss_server* server = nullptr;  // a server drives the context while attached
+    // no-allocation search (hss_search_noalloc): the device address of this
+    // device's scratch array, and launch state whose device buffers are views
+    // into it (events and the pinned read-back word are created with the context)
+    void* scratch = nullptr;
+    hss::LaunchBuffers noalloc;
 
     uint64_t waves() const { return (uint64_t)cus * blocks_per_cu * 4; }
     uint32_t grid_blocks() const { return (uint32_t)(cus * blocks_per_cu); }
@@ -156,5 +161,7 @@ int launch_enqueue_cells(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipS
 // After B.ev[5] has completed: read the misc counters, detect overflow, copy
 // accepted hits on `copy` (a stream with no work that writes B's buffers).
 int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out);
+// The part of launch_collect that allocates nothing: misc counters and timings.
+void launch_read_misc(const LaunchBuffers& B, LaunchResult& out);
 
 }  // namespace hss

@@ -69,4 +69,13 @@ hipError_t launch_batch_reset(const BatchHeader* hdr, const PosDesc* pos, unsign
 hipError_t cell_occupancy(int* blocks_per_cu);
 hipError_t cell_win_occupancy(int* blocks_per_cu);
 
+// Device memory of the no-allocation search (hss_search_noalloc): a static
+// array of this many bytes in every device's copy of the code object.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+// Deferred-leaf capacity of that search (16 bytes each, inside the array).
+static constexpr uint32_t NOALLOC_DEF_CAP = 1u << 20;
+// Device address of that array on `device` (looked up once per device, with
+// the device made current for the lookup and restored afterwards).
+hipError_t noalloc_scratch(int device, void** ptr);
+
 }  // namespace hss

@@ -32,6 +32,7 @@
 // integer arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "hss_build.h"
@@ -1174,6 +1175,39 @@ hipError_t cell_occupancy(int* blocks_per_cu) {
 
 hipError_t cell_win_occupancy(int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, cell_win_kernel, WG, 0);
+}
+
+// ------------------------------------------------ no-allocation scratch
+//
+// Every per-launch buffer of hss_search_noalloc (plan, left sets, tiles,
+// counters, deferred leaves, hits) is carved from this array, so that entry
+// point allocates no device memory.  Each device holds its own copy for the
+// life of the process.  In host code `scratch` names a host-side stand-in:
+// only the address hipGetSymbolAddress returns may reach a kernel or a copy.
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    static constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static void* addr[MAX_DEVICES] = {};
+    if (!ptr || device < 0 || device >= MAX_DEVICES) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> guard(mu);
+    if (!addr[device]) {
+        int current = 0;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        const hipError_t back = hipSetDevice(current);
+        if (e != hipSuccess) return e;
+        if (back != hipSuccess) return back;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 }  // namespace hss

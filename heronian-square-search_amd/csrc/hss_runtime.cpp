@@ -96,6 +96,14 @@ void LaunchBuffers::release() {
     }
 }
 
+// Forget device buffers that are views into memory B does not own (the
+// no-allocation search's scratch array), so that B.release() frees none of it.
+void clear_views(LaunchBuffers& B) {
+    for (DevBuf* d : {&B.header, &B.pos, &B.lefts, &B.builds, &B.tiles, &B.wtiles, &B.sorted, &B.perm, &B.btab,
+                      &B.lfast, &B.lre, &B.lim, &B.misc, &B.def, &B.raw, &B.hits})
+        *d = DevBuf();
+}
+
 }  // namespace hss
 
 namespace {
@@ -143,6 +151,8 @@ extern "C" void hss_ctx_destroy(hss_ctx* c) {
     (void)hipFree(c->d_vre);
     (void)hipFree(c->d_vim);
     c->sync.release();
+    clear_views(c->noalloc);  // views into the scratch array: nothing to free
+    c->noalloc.release();
     c->counters.release();
     if (c->capture) (void)hipStreamDestroy(c->capture);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -268,7 +278,10 @@ extern "C" int hss_ctx_create(uint64_t n_max, int device, hss_ctx** out) {
         hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(HSS_EDEVICE, "stream create failed"));
     c->use_graphs = std::getenv("HSS_NO_GRAPH") == nullptr && !verbose();
-    if (c->sync.create() != hipSuccess) return bail(fail(HSS_EDEVICE, "launch buffers: event or pinned memory"));
+    if (c->sync.create() != hipSuccess || c->noalloc.create() != hipSuccess)
+        return bail(fail(HSS_EDEVICE, "launch buffers: event or pinned memory"));
+    if (noalloc_scratch(device, &c->scratch) != hipSuccess)
+        return bail(fail(HSS_EDEVICE, "address of the no-allocation scratch array"));
     int bpc = 0;
     if (cell_occupancy(&bpc) != hipSuccess || bpc < 1) bpc = 1;
     c->blocks_per_cu = bpc;
@@ -699,7 +712,7 @@ int launch_enqueue_cells(hss_ctx* c, LaunchBuffers& B, const Counters& cnt, hipS
     return run_stages(c, B, cell_params(c, B, cnt), cnt, 1, st);
 }
 
-int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out) {
+void launch_read_misc(const LaunchBuffers& B, LaunchResult& out) {
     out = LaunchResult();
     const uint32_t* m4 = B.h_misc.as<uint32_t>();
     out.deferred = m4[1];
@@ -713,6 +726,11 @@ int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult&
     (void)hipEventElapsedTime(&out.plain_ms, B.ev[2], B.ev[3]);
     (void)hipEventElapsedTime(&out.exact_ms, B.ev[3], B.ev[4]);
     out.cells_ms = out.win_ms + out.plain_ms;
+}
+
+int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult& out) {
+    launch_read_misc(B, out);
+    const uint32_t* m4 = B.h_misc.as<uint32_t>();
     if (m4[1] > B.def_cap) {
         // deferred buffer overflowed: the counts are incomplete; grow for a rerun
         if (m4[1] > (1u << 30)) return fail(HSS_ENOMEM, "deferred-leaf buffer overflow");
@@ -733,6 +751,11 @@ int launch_collect(hss_ctx* c, LaunchBuffers& B, hipStream_t copy, LaunchResult&
 }
 
 }  // namespace hss
+
+static int finish_search(hss_ctx* c, const LaunchBuffers& B, const LaunchResult& R,
+                         const std::vector<unsigned long long>& h, uint32_t n_roots, hss_root_result* per_root,
+                         uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap, uint32_t* n_hits,
+                         std::chrono::steady_clock::time_point t0);
 
 extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
                           hss_root_result* per_root, uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap,
@@ -776,6 +799,15 @@ extern "C" int hss_search(hss_ctx* c, uint64_t limit, const uint64_t* roots, uin
     std::vector<unsigned long long> h((size_t)n_roots * 3);
     HIP_TRY(hipMemcpyAsync(h.data(), c->counters.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return finish_search(c, B, R, h, n_roots, per_root, candidates_total, hits, hit_cap, n_hits, t0);
+}
+
+// Per-root counters, statistics and the de-duplicated, sorted hit list of a
+// completed search launch (shared by hss_search and hss_search_noalloc).
+static int finish_search(hss_ctx* c, const LaunchBuffers& B, const LaunchResult& R,
+                         const std::vector<unsigned long long>& h, uint32_t n_roots, hss_root_result* per_root,
+                         uint64_t* candidates_total, hss_hit* hits, uint32_t hit_cap, uint32_t* n_hits,
+                         std::chrono::steady_clock::time_point t0) {
     uint64_t total = 0, leaves = 0;
     for (uint32_t s = 0; s < n_roots; ++s) {
         total += h[s];
@@ -824,6 +856,138 @@ extern "C" int hss_search_interval(hss_ctx* c, uint64_t limit, uint64_t first, u
     for (uint64_t i = 0; i < roots.size(); ++i) roots[i] = first + i;
     return hss_search(c, limit, roots.data(), (uint32_t)roots.size(), nullptr, candidates_total, hits, hit_cap,
                       n_hits);
+}
+
+namespace {
+
+// Point every device buffer of B into the scratch array, in order, each on a
+// 256-byte boundary; false when the batch does not fit.  The views are never
+// released (clear_views drops them before B.release()).
+bool carve_views(hss_ctx* c, LaunchBuffers& B, uint32_t n_roots, Counters& cnt, size_t& need) {
+    const Plan& P = B.plan;
+    char* const base = static_cast<char*>(c->scratch);
+    size_t used = 0;
+    bool fits = true;
+    auto take = [&](size_t bytes) -> char* {
+        const size_t at = (used + 255) & ~(size_t)255;
+        used = at + std::max<size_t>(bytes, 1);
+        if (used > NOALLOC_SCRATCH_BYTES) fits = false;
+        return fits ? base + at : nullptr;
+    };
+    auto view = [&](DevBuf& d, size_t bytes) {
+        d.p = take(bytes);
+        d.bytes = d.p ? bytes : 0;
+    };
+    const size_t entries = std::max<uint64_t>(P.left_entries, 1);
+    view(B.header, sizeof(BatchHeader));
+    view(B.misc, 64);
+    view(B.pos, (size_t)n_roots * sizeof(PosDesc));
+    view(B.lefts, P.lefts.size() * sizeof(LeftDesc));
+    view(B.builds, P.lefts.size() * sizeof(LeftBuild));
+    view(B.sorted, P.sorted.size() * 4);
+    view(B.btab, P.sorted.size() * (WIN_NB + 1) * 4);
+    view(B.perm, entries * 4);
+    view(B.lfast, (entries + WIN_UNIT) * sizeof(LeftFast));  // padding read by window units
+    view(B.lre, entries * sizeof(i128));
+    view(B.lim, entries * sizeof(i128));
+    view(B.tiles, P.tiles.size() * sizeof(TileDesc));
+    view(B.wtiles, P.wtiles.size() * sizeof(TileDesc));
+    view(B.raw, LaunchBuffers::RAW_CAP * sizeof(HitOut));
+    view(B.hits, LaunchBuffers::HIT_CAP * sizeof(HitOut));
+    B.def_cap = NOALLOC_DEF_CAP;
+    view(B.def, (size_t)B.def_cap * sizeof(DefItem));
+    cnt.cand = reinterpret_cast<unsigned long long*>(take((size_t)n_roots * 3 * 8));
+    cnt.leaves = cnt.cand ? cnt.cand + n_roots : nullptr;
+    cnt.pairs = cnt.cand ? cnt.cand + 2 * (size_t)n_roots : nullptr;
+    need = used;
+    return fits;
+}
+
+}  // namespace
+
+// The search of hss_search with no device or pinned-host allocation: the
+// plan is copied straight from host memory into views of the scratch array,
+// the kernel chain is launched directly (no graph capture) and the deferred
+// buffer has a fixed capacity.
+extern "C" int hss_search_noalloc(hss_ctx* c, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
+                                  hss_root_result* per_root, uint64_t* candidates_total, hss_hit* hits,
+                                  uint32_t hit_cap, uint32_t* n_hits) {
+    if (!c) return fail(HSS_EINVAL, "null context");
+    if (c->server) return fail(HSS_EINVAL, "context is attached to a server");
+    if (n_roots && !roots) return fail(HSS_EINVAL, "roots is null");
+    if (hit_cap && !hits) return fail(HSS_EINVAL, "hits is null");
+    if (!c->scratch) return fail(HSS_EDEVICE, "no scratch array on this device");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(HSS_EDEVICE, "hipSetDevice failed");
+    c->stats = hss_stats{};
+    if (n_hits) *n_hits = 0;
+    if (candidates_total) *candidates_total = 0;
+    if (!n_roots) return HSS_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<PosReq> reqs(n_roots);
+    for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limit, s};
+    LaunchBuffers& B = c->noalloc;
+    int rc = launch_plan(c, B, reqs.data(), n_roots);
+    if (rc != HSS_OK) return rc;
+    c->stats.plan_ms = ms_since(t0);
+    Counters cnt;
+    size_t need = 0;
+    if (!carve_views(c, B, n_roots, cnt, need)) {
+        clear_views(B);
+        return fail(HSS_ENOMEM, "batch needs " + std::to_string(need) + " bytes of scratch; the no-allocation "
+                                "search has " + std::to_string(NOALLOC_SCRATCH_BYTES) + ": split the batch");
+    }
+    struct Drop {
+        LaunchBuffers& b;
+        ~Drop() { clear_views(b); }
+    } drop{B};
+    const Plan& P = B.plan;
+    const hipStream_t st = c->stream;
+    const BatchHeader hdr = batch_header(B, true);
+    HIP_TRY(hipEventRecord(B.ev[0], st));
+    HIP_TRY(hipMemcpyAsync(B.header.p, &hdr, sizeof hdr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(B.pos.p, P.pos.data(), n_roots * sizeof(PosDesc), hipMemcpyHostToDevice, st));
+    if (!P.lefts.empty()) {
+        HIP_TRY(hipMemcpyAsync(B.lefts.p, P.lefts.data(), P.lefts.size() * sizeof(LeftDesc), hipMemcpyHostToDevice,
+                               st));
+        HIP_TRY(hipMemcpyAsync(B.builds.p, P.builds.data(), P.builds.size() * sizeof(LeftBuild),
+                               hipMemcpyHostToDevice, st));
+    }
+    if (!P.sorted.empty())
+        HIP_TRY(hipMemcpyAsync(B.sorted.p, P.sorted.data(), P.sorted.size() * 4, hipMemcpyHostToDevice, st));
+    if (!P.tiles.empty())
+        HIP_TRY(hipMemcpyAsync(B.tiles.p, P.tiles.data(), P.tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
+                               st));
+    if (!P.wtiles.empty())
+        HIP_TRY(hipMemcpyAsync(B.wtiles.p, P.wtiles.data(), P.wtiles.size() * sizeof(TileDesc),
+                               hipMemcpyHostToDevice, st));
+    const CellParams K = cell_params(c, B, cnt);
+    for (int stage = 0; stage < LaunchBuffers::STAGES; ++stage) {
+        const bool empty = (stage == 1 && P.wtiles.empty()) || (stage == 2 && P.tiles.empty());
+        if (!empty) {
+            rc = enqueue_stage(c, B, K, stage, st);
+            if (rc != HSS_OK) return rc;
+        }
+        HIP_TRY(hipEventRecord(B.ev[1 + stage], st));
+        if (stage == 0) HIP_TRY(hipEventRecord(B.ev[6], st));
+    }
+    B.replayed = false;
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(B.ev[5], st));
+    HIP_TRY(hipEventSynchronize(B.ev[5]));
+    LaunchResult R;
+    launch_read_misc(B, R);
+    const uint32_t* m4 = B.h_misc.as<uint32_t>();
+    if (m4[1] > B.def_cap)
+        return fail(HSS_ENOMEM, "deferred-leaf buffer of the no-allocation search overflowed (" +
+                                    std::to_string(m4[1]) + " > " + std::to_string(B.def_cap) + "): split the batch");
+    if (m4[2] > LaunchBuffers::RAW_CAP) return fail(HSS_EOVERFLOW, "raw fourth-power buffer overflow");
+    if (m4[3] > LaunchBuffers::HIT_CAP) return fail(HSS_EOVERFLOW, "accepted-hit buffer overflow");
+    R.hits.resize(m4[3]);
+    if (m4[3]) HIP_TRY(hipMemcpyAsync(R.hits.data(), B.hits.p, m4[3] * sizeof(HitOut), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> h((size_t)n_roots * 3);
+    HIP_TRY(hipMemcpyAsync(h.data(), cnt.cand, h.size() * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return finish_search(c, B, R, h, n_roots, per_root, candidates_total, hits, hit_cap, n_hits, t0);
 }
 
 extern "C" uint32_t hss_gaussian_weight(uint64_t l) {

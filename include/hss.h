@@ -114,6 +114,25 @@ int hss_search_interval(hss_ctx* ctx, uint64_t limit, uint64_t first, uint64_t l
                         uint32_t* n_hits);
 
 /*
+ * hss_search without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory during a call (code under graph
+ * capture, tight serving loops, harnesses that forbid it).  Same arguments,
+ * results and errors as hss_search, but every per-call device buffer (plan,
+ * left sets, tiles, counters, deferred leaves, hits) lives in one static
+ * 256 MiB array per device, whose address the context looks up when it is
+ * created; the call allocates no device or pinned host memory and launches
+ * the kernels directly (no graph).  A batch whose buffers exceed the array, or
+ * whose deferred leaves exceed its fixed capacity (2^20), fails with
+ * HSS_ENOMEM and a message to split it; a batch of 512 root positions near
+ * 7.5e6 at LIMIT 1e7 needs about 70 MiB.  Every call on a device shares that
+ * device's array: calls that overlap on one device (two contexts, two
+ * threads) race and must be serialised by the caller.
+ */
+int hss_search_noalloc(hss_ctx* ctx, uint64_t limit, const uint64_t* roots, uint32_t n_roots,
+                       hss_root_result* per_root, uint64_t* candidates_total, hss_hit* hits,
+                       uint32_t hit_cap, uint32_t* n_hits);
+
+/*
  * Scheduling knobs (no counterpart in the reference, whose kernel has one
  * fixed launch shape, src/gpu_search_v42.cu:457-458), keyed on Gaussian
  * weight.  A tile -- one left set of a root against a range of partners --
