@@ -66,6 +66,24 @@ HSS_HD inline GInt gauss_value(int64_t a, int64_t b, uint32_t E, uint32_t r) {
     return gmul(gpow(pi, r), gpow(pc, 4 * E - r));
 }
 
+// Area class of a Gaussian value pi^r conj(pi)^(4E - r) of the split prime p
+// (the class rule of hss_math.h): bit 0 is set when p = 2 (mod 3) and r is
+// odd, bit 1 when p = 2 or 3 (mod 5) and r is odd.  The class of a product of
+// values is the xor of its factors' classes; the central value (r = 2E) has
+// class 0.
+HSS_HD inline uint32_t value_class(uint64_t p, uint32_t r) {
+    const uint32_t odd = r & 1u;
+    const uint32_t p3 = (uint32_t)(p % 3), p5 = (uint32_t)(p % 5);
+    return (p3 == 2 ? odd : 0u) | ((p5 == 2 || p5 == 3) ? odd << 1 : 0u);
+}
+
+// class of the product of values with digits dig over primes p[0..k)
+HSS_HD inline uint32_t digits_class(uint32_t k, const uint32_t* p, const uint32_t* dig) {
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < k; ++j) c ^= value_class(p[j], dig[j]);
+    return c;
+}
+
 // offset of exponent E inside a prime's value block: sum_{e<E} (4e+1)
 HSS_HD inline uint32_t exp_offset(uint32_t E) { return (E - 1) * (2 * E + 1); }
 

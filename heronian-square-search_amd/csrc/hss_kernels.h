@@ -20,7 +20,7 @@ struct CellParams {
     const uint32_t* off;
     const double* cre;
     const double* cim;
-    const uint32_t* cres;  // (re mod 180) | (im mod 180) << 16
+    const uint32_t* cres;  // (re mod 180) | class << 8 | (im mod 180) << 16
     // batch
     const PosDesc* pos;
     const LeftDesc* lefts;

@@ -211,6 +211,7 @@ __global__ __launch_bounds__(256) void k_fill_cache(const uint32_t* spf, const u
     }
     uint32_t dig[MAX_SPLIT];
     decode_rank(t, k, es, dig);
+    const uint32_t cls = digits_class(k, ps, dig);
     int64_t re = fixed, im = 0;
     for (uint32_t j = 0; j < k; ++j) {
         uint32_t at = vbase[ps[j]] + exp_offset(es[j]) + dig[j];
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(256) void k_fill_cache(const uint32_t* spf, const u
     cre[e] = (double)re;
     cim[e] = (double)im;
     int64_t r1 = re % (int64_t)QA, i1 = im % (int64_t)QA;
-    cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
+    cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | cls << CRES_CLASS_SHIFT | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
 }
 
 // Gaussian value of rank t of a left set: fixed * prod of the merged prime
@@ -265,19 +266,35 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
     }
 }
 
+// Class of the entry of rank t of a left set (the class rule of hss_math.h).
+__device__ inline uint32_t left_class(const LeftBuild& b, uint32_t t) {
+    uint32_t dig[MAX_SPLIT];
+    decode_rank(t, b.k, b.E, dig);
+    return digits_class(b.k, b.p, dig);
+}
+
+// Bucket of a sorted left set's entry: its area group's slice of the WIN_NB
+// buckets (WIN_NB >> group bits each), then its angle theta modulo pi.
+__device__ inline uint32_t sort_bucket(uint32_t group, uint32_t gbits, float key) {
+    const uint32_t nbg = WIN_NB >> gbits;
+    return group * nbg + theta_bucket((double)key, nbg);
+}
+
 // One block per sorted left set (grid-stride over hdr->nsorted): order the
-// non-central entries by psi (angle modulo pi/2, from the same FP64
-// coordinates k_build_left stores), write the permutation and the bucket
-// table bt[b] = number of entries with psi_bucket < b, b = 0..WIN_NB.  A
-// counting sort on the buckets (whose starts are that table) followed by an
-// insertion sort of each bucket by (key, index): entries of one left set share
-// their norm, so their angles are distinct and spread over the buckets (a few
-// per bucket), and the order equals a full sort by (key, index).
+// non-central entries by (area group, theta) -- the group packs the entry's
+// class bits that the left set's pairs can constrain (left_kmask), theta is the
+// angle modulo pi from the same FP64 coordinates k_build_left stores -- write
+// the permutation and the bucket table bt[b] = number of entries with bucket
+// < b, b = 0..WIN_NB.  A counting sort on the buckets (whose starts are that
+// table) followed by an insertion sort of each bucket by (key, index): entries
+// of one left set share their norm, so their angles are distinct and spread
+// over the buckets (a few per bucket), and the order equals a full sort by
+// (group, key, index).
 __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const uint32_t* sorted,
                                                    const LeftBuild* builds, const LeftDesc* lefts,
                                                    const uint32_t* vbase, const i128* vre, const i128* vim,
                                                    uint32_t* perm, uint32_t* btab) {
-    __shared__ float key[WIN_MAX_SORT];       // psi_u rounded to FP32 (the window's error budget allows it)
+    __shared__ float key[WIN_MAX_SORT];       // theta_u rounded to FP32 (the window's error budget allows it)
     __shared__ uint16_t order[WIN_MAX_SORT];  // entry indices in sorted order
     __shared__ uint32_t start[WIN_NB + 1];    // bucket counts, then bucket starts
     __shared__ uint32_t cursor[WIN_NB];
@@ -288,14 +305,16 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
         const uint32_t li = sorted[bi];
         const LeftBuild b = builds[li];
         const LeftDesc L = lefts[li];
+        const uint32_t kmask = left_kmask(L);
+        const uint32_t gbits = __popc(kmask);
         const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
         for (uint32_t bk = tid; bk <= WIN_NB; bk += blockDim.x) start[bk] = 0;
         __syncthreads();
         for (uint32_t t = tid; t < n; t += blockDim.x) {
             const GInt z = left_value(b, t, vbase, vre, vim);
-            const float k = (float)psi_of(i128_to_double(z.re), i128_to_double(z.im));
+            const float k = (float)theta_of(i128_to_double(z.re), i128_to_double(z.im));
             key[t] = k;
-            atomicAdd(&start[psi_bucket<WIN_NB>(k)], 1u);
+            atomicAdd(&start[sort_bucket(area_group(left_class(b, t), kmask), gbits, k)], 1u);
         }
         __syncthreads();
         // exclusive scan of the WIN_NB counts, WIN_NB / 256 per thread
@@ -321,7 +340,9 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
         }
         if (tid == 0) start[WIN_NB] = n;
         __syncthreads();
-        for (uint32_t t = tid; t < n; t += blockDim.x) order[atomicAdd(&cursor[psi_bucket<WIN_NB>(key[t])], 1u)] = (uint16_t)t;
+        for (uint32_t t = tid; t < n; t += blockDim.x)
+            order[atomicAdd(&cursor[sort_bucket(area_group(left_class(b, t), kmask), gbits, key[t])], 1u)] =
+                (uint16_t)t;
         __syncthreads();
         for (uint32_t bk = tid; bk < WIN_NB; bk += blockDim.x) {
             const uint32_t s0 = start[bk], s1 = start[bk + 1];
@@ -616,6 +637,7 @@ struct Partner {
     uint32_t c180, d180, n180;  // c, d and -d mod 180
     uint32_t cF, dF, nF;        // their area factors for the pair's class (area_x)
     uint32_t vidx, kj, jj;
+    uint32_t cls;           // area class of v (the class rule of hss_math.h)
     bool vc;                // v is the central (real) entry of its half-set
     LaneCtx ctx;
 };
@@ -658,7 +680,8 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
     v.c = P.cre[v.vidx];
     v.d = P.cim[v.vidx];
     const uint32_t vr = P.cres[v.vidx];
-    v.c180 = vr & 0xFFFFu;
+    v.c180 = vr & 0xFFu;
+    v.cls = (vr >> CRES_CLASS_SHIFT) & 3u;
     v.d180 = vr >> 16;
     v.n180 = v.d180 ? QA - v.d180 : 0u;
 }
@@ -709,7 +732,7 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
             for (uint32_t j = 0; j < S.ninert; ++j) ok = ok && !divides(k, S.inert_inv[j], S.inert_thr[j]);
             const uint32_t ok0 = P.off[kq];
             uint32_t Hk = P.off[kq + 1] - ok0;
-            Hk = (ok && !(L.trivial && Hk == 1)) ? Hk : 0u;
+            Hk = (ok && !((L.flags & LEFT_TRIVIAL) && Hk == 1)) ? Hk : 0u;
             pairs += Hk ? 1 : 0;
             // leaves of the pair: 2H - 1 per non-central entry of k, H for its central one
             leaves += Hk ? (uint64_t)Hk * (2u * L.H - 1u) - (L.H - 1u) : 0ull;
@@ -831,48 +854,178 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 
 // ------------------------------------------------------ window-mode kernel
 //
-// Tiles whose left set is stored sorted by psi (hss_math.h, range windows).
-// Per chunk of 64 partner entries (one per lane, mapped as in cell_kernel):
-//   phase 1 (lane = partner entry v): the central u leaf as in cell_kernel;
-//     the index ranges of the left entries whose leaf u*v or u*conj(v) can
-//     count, cut into units of WIN_UNIT consecutive entries; the lane's
-//     partner data, fast-path context and ranges go to a per-wave LDS record;
-//   phase 2 (lane = one unit): the wave walks the flattened units of all 64
-//     partners 64 at a time; a lane reads its owner's record once and
-//     evaluates the unit's leaves with leaf_fast and vector loads of u.
-// Leaves outside the ranges count 0 exactly and are skipped.
+// Tiles whose left set is stored sorted by (area group, theta) (hss_math.h:
+// the class rule and orientation windows).  Per chunk of 64 partner entries
+// (one per lane, mapped as in cell_kernel):
+//   phase 1 (lane = partner entry v): the leaf of the left set's central entry
+//     with v (class rule, both orientations); the pair's band, orientation-1
+//     window and the angle of v go to a per-lane LDS record.  v has one
+//     virtual partner w = i^rho v per orientation (rho = 1, orientation 2,
+//     on even pairs only) and class variant j (the class bits the pair leaves
+//     free); the virtual partners of the chunk are numbered by a prefix sum;
+//   phase 2 (lane = virtual partner, 64 at a time): the index ranges of the
+//     one area group whose leaves u w and u conj(w) pass the area tests,
+//     inside the orientation-1 window shifted by the angle of w, cut into
+//     units of WIN_UNIT entries (a second per-lane LDS record);
+//   phase 3 (lane = one unit): the flattened units of those 64 virtual
+//     partners, 64 at a time; a lane evaluates its unit's leaves: X = Re(u w),
+//     the FP64 band of orientation 1 and the root test of each counted term.
+// Leaves outside the ranges count 0 exactly (wrong orientation, wrong class
+// or outside the window) and are skipped; a leaf inside them with an
+// undecided band goes to the exact stage with its orientation only.
 
-// 104 bytes: four blocks of four waves fit a CU's LDS with the root queues.
-// The narrowed range bounds are derived from the widened ones (n = w -+ tau),
-// the area-test constants from the class of h (3, 9, 15, 45).
-struct WinRec {
-    double c, d;  // partner v
-    double mid, half_w, h2_w, tau;  // h2_w holds l^4 + m^4 on one-term tiles
-    uint32_t cres;    // c mod 180 | (d mod 180) << 8 | partner entry << 16
-    uint32_t kjh;     // k | class of h << 28
-    uint16_t st[8];   // range starts: 0-3 leaves u*v, 4-7 leaves u*conj(v)
-    uint16_t en[8];   // range ends
-    uint16_t ucum[8]; // units before each range
+// Partner entry of a chunk (phase 1 -> 2, 3).  Records are 16-byte aligned
+// and a multiple of 16 bytes long, so that every wide LDS access is aligned.
+struct alignas(16) WinPart {
+    double c, d;                   // v
+    double mid, half_w, h2_w, tau; // the pair's band (hss_math.h Band; l^4 + m^4 is recomputed from k)
+    double wa, wla, wb;            // orientation-1 window arcs (Window1; two arcs have equal lengths)
+    double thv;                    // angle of v modulo pi
+    uint32_t vidx;                 // v's cache entry
+    uint32_t kj;                   // k
+    uint32_t info;                 // class | cm << 2 | even << 4 | real v << 5 | window arcs << 6
+    uint32_t pad;
 };
-static_assert(sizeof(WinRec) == 104, "LDS budget of cell_win_kernel");
+static_assert(sizeof(WinPart) == 96, "LDS budget of cell_win_kernel");
 
-static constexpr uint32_t AREA_INV[4] = {inv_mod_2_24(3), inv_mod_2_24(9), inv_mod_2_24(15), inv_mod_2_24(45)};
-static constexpr uint32_t AREA_THR[4] = {0x3FFFFFu / 3, 0x3FFFFFu / 9, 0x3FFFFFu / 15, 0x3FFFFFu / 45};
+// Virtual partner of a batch (phase 2 -> 3): its ranges and partner lane.
+// 48 + 96 bytes per lane: four blocks of four waves fit a CU's 160 KB of LDS.
+struct alignas(16) WinVp {
+    uint32_t se[8];  // range p (0-3 leaves u w, 4-7 u conj(w)): start | end << 16 (left-set entry indices)
+    uint32_t uc[4];  // units before ranges 2i (low half) and 2i + 1; range 0's (always 0) holds
+                     // the partner lane | rho << 6
+};
+static_assert(sizeof(WinVp) == 48, "LDS budget of cell_win_kernel");
+
+// The leaf c_u * v of the left set's central (real) entry with partner entry
+// v by the class rule: orientation 1 (d = c_u Re v) counts only when v's class
+// is 0 on the constrained bits cm, orientation 2 (d = c_u Im v) only on even
+// pairs with class 3 on cm.  A real v makes the leaf real: orientation 1 has
+// area 0, orientation 2 has d = 0 and counts once when lo <= 0.  Counted terms
+// are root-tested here; an undecided leaf goes whole to the exact stage.
+template <bool ONE>
+__device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
+                                   const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t& cnt) {
+    const double X = cu * v.c, Y = cu * v.d;
+    const bool ok1 = act && !v.vc && (v.cls & cm) == 0u;
+    const bool ok2 = act && even && ((v.cls ^ 3u) & cm) == 0u;
+    const bool real2 = ok2 && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
+    const double h2n = b.h2_w - b.tau;
+    bool u1, u2;
+    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
+    const uint32_t f2 = o1_decide<ONE>(ok2 && !v.vc, Y, b, u2);
+    const bool und = u1 | u2 | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
+    if (__builtin_amdgcn_ballot_w64(und)) push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | DEF_FULL});
+    const uint32_t f = und ? 0u : (f1 | f2 << 2);
+    cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
+    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
+    bool pass = false;
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+        if (ONE && (t & 1u)) continue;
+        const bool on = (f >> t) & 1u;
+        if (!__builtin_amdgcn_ballot_w64(on)) continue;
+        const double a = t < 2u ? ax : ay;
+        pass |= on && root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, a, b.lm));
+    }
+    if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
+}
+
+// Phase 3: the flattened units of a batch of virtual partners.
+template <bool ONE>
+__device__ inline void win_units(const CellParams& P, const TileCtx& tc, const LeftFast* lf, const WinPart* part,
+                                 const WinVp* vps, uint32_t* mark, uint32_t uex, uint32_t units, uint32_t nunits,
+                                 uint32_t& cnt) {
+    const uint32_t lane = __lane_id();
+    uint32_t carry = 0;
+    for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
+        const uint32_t wi = w0 + lane;
+        const uint32_t o = step_owner(mark, uex, units, w0, carry);
+        const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
+        const bool liveu = wi < nunits;
+        const uint32_t wq = liveu ? wi - first : 0u;
+        const WinVp& Q = *(const WinVp*)((const char*)vps + __umul24(o, (uint32_t)sizeof(WinVp)));
+        uint32_t uc[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t w = Q.uc[i];
+            uc[2 * i] = w & 0xFFFFu;
+            uc[2 * i + 1] = w >> 16;
+        }
+        uint32_t pc = 0;
+#pragma unroll
+        for (int p = 1; p < 8; ++p) pc += uc[p] <= wq ? 1u : 0u;
+        const uint32_t pr = uc[0];
+        const uint32_t se = Q.se[pc];
+        const uint32_t ucp = pc ? (Q.uc[pc >> 1] >> (16u * (pc & 1u))) & 0xFFFFu : 0u;
+        const uint32_t ui0 = (se & 0xFFFFu) + (wq - ucp) * WIN_UNIT;
+        const uint32_t uend = liveu ? se >> 16 : 0u;
+        const bool cj = pc >= 4;
+        const WinPart& V = *(const WinPart*)((const char*)part + __umul24(pr & 63u, (uint32_t)sizeof(WinPart)));
+        const bool rho = (pr >> 6) & 1u;
+        // w = i^rho v; u conj(w) is u w with Im w negated: one sign per unit, none per leaf
+        const double vc = V.c, vd = V.d;
+        const double ce = rho ? -vd : vc, dw = rho ? vc : vd;
+        const double de = cj ? -dw : dw;
+        const Band b{V.mid, V.half_w, V.h2_w, V.tau, lm_of(tc.q, V.kj, tc.l4_d)};
+        const uint32_t kinfo = V.kj | (rho ? DEF_O2 : DEF_O1) | (cj ? DEF_CONJ : 0u);
+        const uint32_t vidx = V.vidx;
+        const uint32_t nlive = uend > ui0 ? uend - ui0 : 0u;  // live entries of the unit
+        // the unit's entries from one base: ui0 <= nsort, and the left entries
+        // carry WIN_UNIT entries of padding, so every load is in bounds;
+        // entries past uend are masked by live
+        const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
+        double AX[WIN_UNIT];
+        uint32_t F[WIN_UNIT];
+        bool U[WIN_UNIT];
+#pragma unroll
+        for (uint32_t s = 0; s < WIN_UNIT; ++s) {
+            const LeftFast u = up[s];
+            const double X = __builtin_fma(u.re, ce, -(u.im * de));
+            AX[s] = __builtin_fabs(X);
+            F[s] = o1_decide<ONE>(s < nlive, X, b, U[s]);
+        }
+        bool uany = false;
+#pragma unroll
+        for (uint32_t s = 0; s < WIN_UNIT; ++s) uany |= U[s];
+        if (__builtin_amdgcn_ballot_w64(uany)) {  // rare
+#pragma unroll
+            for (uint32_t s = 0; s < WIN_UNIT; ++s)
+                push_global(P, U[s], DefItem{tc.left, ui0 + s, vidx, kinfo | DEF_FULL});
+        }
+        bool pass[WIN_UNIT];
+        bool pany = false;
+#pragma unroll
+        for (uint32_t s = 0; s < WIN_UNIT; ++s) {
+            cnt += (uint32_t)__popc(F[s]);
+            pass[s] = (F[s] & 1u) && root_near_integer(__builtin_fma(-2.0, AX[s], b.lm));
+        }
+        if constexpr (!ONE) {
+            bool any2 = false;
+#pragma unroll
+            for (uint32_t s = 0; s < WIN_UNIT; ++s) any2 |= (F[s] & 2u) != 0u;
+            if (__builtin_amdgcn_ballot_w64(any2)) {
+#pragma unroll
+                for (uint32_t s = 0; s < WIN_UNIT; ++s)
+                    pass[s] |= (F[s] & 2u) && root_near_integer(__builtin_fma(2.0, AX[s], b.lm));
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < WIN_UNIT; ++s) pany |= pass[s];
+        if (__builtin_amdgcn_ballot_w64(pany)) {  // rare: near-integer fourth roots
+#pragma unroll
+            for (uint32_t s = 0; s < WIN_UNIT; ++s) push_global(P, pass[s], DefItem{tc.left, ui0 + s, vidx, kinfo});
+        }
+    }
+}
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
-    __shared__ QEntry s_q[WAVES][QCAP];
-    __shared__ WinRec s_rec[WAVES][64];
+    __shared__ WinPart s_part[WAVES][64];
+    __shared__ WinVp s_vp[WAVES][64];
     __shared__ uint32_t s_mark[WAVES][65];
-    __shared__ uint2 s_area[4];  // (inv24, thr22) per class of h
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (threadIdx.x < 4) {
-        const uint32_t i = threadIdx.x;
-        s_area[i] = make_uint2(i == 0 ? AREA_INV[0] : i == 1 ? AREA_INV[1] : i == 2 ? AREA_INV[2] : AREA_INV[3],
-                               i == 0 ? AREA_THR[0] : i == 1 ? AREA_THR[1] : i == 2 ? AREA_THR[2] : AREA_THR[3]);
-    }
-    __syncthreads();
-    RootQueue rq{s_q[wave], 0, 0, 0};
-    WinRec* rec = s_rec[wave];
+    WinPart* part = s_part[wave];
+    WinVp* vps = s_vp[wave];
     uint32_t* mark = s_mark[wave];
     for (;;) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_wtile, lane == 0 ? 1u : 0u));
@@ -884,6 +1037,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const LeftFast* lf = lfast + L.off;
         const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
         const uint32_t nsort = L.H - 1;
+        const uint32_t kmask = left_kmask(L);
+        const uint32_t nbg = WIN_NB >> __popc(kmask);
+        const double cu = lf[nsort].re;  // the central entry is real
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
@@ -893,131 +1049,100 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
             pair_ctx(L, S, kj, v.ctx);
-            partner_factors(v);
-            // one-term tiles queue in the ZZ form (l^4 + m^4 is v.ctx.lm)
+            const Band b{v.ctx.mid, v.ctx.half_w, v.ctx.h2_w, v.ctx.tau, v.ctx.lm};
+            // the pair's constrained class bits (3 | m and 5 | m by inverse multiplication)
+            const bool m3z = L.q3 == 0u || kj * 0xAAAAAAABu <= 0x55555555u;
+            const bool m5z = L.q5 == 0u || kj * 0xCCCCCCCDu <= 0x33333333u;
+            const uint32_t cm = area_constrained(S.l3 == 0u || m3z, S.l5 != 0u && !m5z);
+            const bool even = S.l_even || !(kj & 1u);  // m = q k with q odd
             if (one)
-                central_leaf<true>(P, rq, tc, act, lf[nsort], nsort, v, cnt);
+                central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, cnt);
             else
-                central_leaf<false>(P, rq, tc, act, lf[nsort], nsort, v, cnt);
-            // ranges of non-central entries
+                central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, cnt);
+            evald += act ? 1u : 0u;
             const double md = (double)L.q * (double)kj;  // exact: m < 2^26
             const double m2 = md * md;                      // exact: m^2 < 2^52
-            Window w;
-            make_window((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, v.ctx.tau, 2.0 * S.l2_d * m2, w);
-            const double pv = psi_of_f(v.c, v.d);
-            uint32_t rs[8], re[8];
-            const uint32_t n0 = window_ranges<WIN_NB>(w, bt, nsort, pv, false, rs, re);
-            const uint32_t n1 = window_ranges<WIN_NB>(w, bt, nsort, pv, true, rs + 4, re + 4);
-            const bool live0 = act && n0 != 0, live1 = act && !v.vc && n1 != 0;
+            Window1 w;
+            make_window_o1((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, b.tau, 2.0 * S.l2_d * m2, w);
+            const uint32_t nvp = act ? allowed_count(cm, kmask) << (even ? 1 : 0) : 0u;
             __builtin_amdgcn_wave_barrier();
-            WinRec& R = rec[lane];
-            uint32_t items = 0, units = 0;
-            for (int p = 0; p < 8; ++p) {
-                const uint32_t len = (p < 4 ? live0 : live1) ? re[p] - rs[p] : 0u;
-                R.st[p] = (uint16_t)rs[p];
-                R.en[p] = (uint16_t)(rs[p] + len);
-                R.ucum[p] = (uint16_t)units;
-                items += len;
-                units += (len + WIN_UNIT - 1) / WIN_UNIT;
+            {
+                WinPart& R = part[lane];
+                R.c = v.c;
+                R.d = v.d;
+                R.mid = b.mid;
+                R.half_w = b.half_w;
+                R.h2_w = b.h2_w;
+                R.tau = b.tau;
+                R.wa = w.a;
+                R.wla = w.la;
+                R.wb = w.b;
+                R.thv = theta_of_f(v.c, v.d);
+                R.vidx = v.vidx;
+                R.kj = kj;
+                R.info = v.cls | cm << 2 | (even ? 16u : 0u) | (v.vc ? 32u : 0u) | (uint32_t)(w.n & 3) << 6;
+                R.pad = 0;
             }
-            evald += act ? 1u + items : 0u;  // the central leaf and the items
-            const uint32_t inv = v.ctx.inv24;
-            const uint32_t hcl = inv == AREA_INV[0] ? 0u : inv == AREA_INV[1] ? 1u : inv == AREA_INV[2] ? 2u : 3u;
-            R.c = v.c;
-            R.d = v.d;
-            R.mid = v.ctx.mid;
-            R.half_w = v.ctx.half_w;
-            R.h2_w = one ? v.ctx.lm : v.ctx.h2_w;  // one-term tiles need l^4 + m^4 instead
-            R.tau = v.ctx.tau;
-            R.cres = v.c180 | v.d180 << 8 | jj << 16;
-            R.kjh = kj | hcl << 28;
-            const uint32_t ucl = wave_incl_scan(units);
-            const uint32_t uex = ucl - units;  // first unit of this lane's partner
-            const uint32_t nunits = __builtin_amdgcn_readlane(ucl, 63);
+            const uint32_t vincl = wave_incl_scan(nvp);
+            const uint32_t vex = vincl - nvp;
+            const uint32_t nvps = __builtin_amdgcn_readlane(vincl, 63);
             __builtin_amdgcn_wave_barrier();
-            // ---- phase 2: lane = unit of WIN_UNIT consecutive entries of one range
-            // (instantiated for tiles with and without second-sign terms)
-            const auto phase2 = [&](auto one_c) {
-                constexpr bool ONE = decltype(one_c)::value;
-                uint32_t carry = 0;
-                for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
-                    const uint32_t wi = w0 + (uint32_t)lane;
-                    const uint32_t o = step_owner(mark, uex, units, w0, carry);
-                    const uint32_t first = (uint32_t)__shfl((int)uex, (int)o, 64);
-                    const bool liveu = wi < nunits;
-                    const uint32_t wq = liveu ? wi - first : 0u;
-                    const WinRec& Q = *(const WinRec*)((const char*)rec + __umul24(o, (uint32_t)sizeof(WinRec)));
-                    uint32_t pc = 0;
-                    for (int p = 1; p < 8; ++p) pc += Q.ucum[p] <= wq ? 1u : 0u;
-                    const uint32_t ui0 = (uint32_t)Q.st[pc] + (wq - (uint32_t)Q.ucum[pc]) * WIN_UNIT;
-                    const uint32_t uend = liveu ? (uint32_t)Q.en[pc] : 0u;
-                    const bool cj = pc >= 4;
-                    // u*conj(v) is u*v with d negated: one sign per unit, none per leaf
-                    const double qc = Q.c, qd = cj ? -Q.d : Q.d;
-                    const uint32_t qr = Q.cres;
-                    // residues of the unit's c and d (-d for u*conj(v)) and of minus that d
-                    const uint32_t qc180 = qr & 0xFFu, qd180 = (qr >> 8) & 0xFFu;
-                    const uint32_t qn180 = qd180 ? QA - qd180 : 0u;
-                    const uint32_t qp = cj ? qn180 : qd180, qn = cj ? qd180 : qn180;
-                    const uint32_t qkjh = Q.kjh, qh = qkjh >> 28;
-                    LaneCtx qx;
-                    qx.mid = Q.mid;
-                    qx.half_w = Q.half_w;
-                    qx.h2_w = Q.h2_w;  // l^4 + m^4 on one-term tiles (unused by leaf_decide<true>)
-                    qx.tau = Q.tau;
-                    qx.half_n = qx.half_w - qx.tau;
-                    qx.h2_n = qx.h2_w - qx.tau;
-                    const uint2 ar = s_area[qh];
-                    qx.inv24 = ar.x;
-                    qx.thr22 = ar.y;
-                    // their area factors (area_x)
-                    const uint32_t cF = mul_u24(qc180, ar.x), pF = mul_u24(qp, ar.x), nF = mul_u24(qn, ar.x);
-                    const uint32_t kq = (qkjh & 0x7FFFFFFu) | (cj ? 1u << 27 : 0u);
-                    // ids pack_ui(ui0 + s, partner entry): ui0 + s stays below 2^ID_U_BITS
-                    const uint32_t ub = ui0 | (qr >> 16) << ID_U_BITS;
-                    double UX[WIN_UNIT], UY[WIN_UNIT];
-                    uint32_t UF[WIN_UNIT];
-                    bool UU[WIN_UNIT];
-                    const uint32_t nlive = uend > ui0 ? uend - ui0 : 0u;  // live entries of the unit
-                    // the unit's entries from one base: ui0 <= nsort, and the left
-                    // entries carry WIN_UNIT entries of padding, so every load is in
-                    // bounds; entries past uend are masked by live
-                    const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
+            // ---- phase 2: lane = virtual partner, 64 at a time
+            uint32_t vcarry = 0;
+            for (uint32_t v0 = 0; v0 < nvps; v0 += 64) {
+                const uint32_t vi = v0 + (uint32_t)lane;
+                const uint32_t o = step_owner(mark, vex, nvp, v0, vcarry);
+                const bool livev = vi < nvps;
+                // every lane executes the cross-lane read (a source lane may not be live)
+                const uint32_t ovex = (uint32_t)__shfl((int)vex, (int)o, 64);
+                const uint32_t q = livev ? vi - ovex : 0u;
+                const WinPart& V = *(const WinPart*)((const char*)part + __umul24(o, (uint32_t)sizeof(WinPart)));
+                const uint32_t info = V.info;
+                const uint32_t vcls = info & 3u, vcm = (info >> 2) & 3u;
+                const bool veven = (info >> 4) & 1u, vreal = (info >> 5) & 1u;
+                const uint32_t rho = veven ? (q & 1u) : 0u, j = veven ? q >> 1 : q;
+                const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, j), kmask);
+                const uint32_t* btg = bt + g * nbg;
+                Window1 vw;
+                vw.a = V.wa;
+                vw.la = V.wla;
+                vw.b = V.wb;
+                vw.lb = V.wla;
+                const uint32_t wn = (info >> 6) & 3u;
+                vw.n = wn == 3u ? -1 : (int)wn;
+                double thw = V.thv + (rho ? HALF_PI : 0.0);
+                thw = thw >= PI_D ? thw - PI_D : thw;
+                uint32_t rs[8], re[8];
+                const uint32_t n0 = window_o1_ranges(vw, btg, nbg, thw, false, rs, re);
+                const uint32_t n1 = window_o1_ranges(vw, btg, nbg, thw, true, rs + 4, re + 4);
+                const bool live0 = livev && n0 != 0, live1 = livev && !vreal && n1 != 0;
+                __builtin_amdgcn_wave_barrier();
+                WinVp& R = vps[lane];
+                uint32_t items = 0, units = 0, ucs[8];
 #pragma unroll
-                    for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-                        const bool live = s < nlive;
-                        const LeftFast u = up[s];
-                        const double t1 = u.im * qd, t2 = u.re * qd;
-                        UX[s] = __builtin_fma(u.re, qc, -t1);
-                        UY[s] = __builtin_fma(u.im, qc, t2);
-                        const uint32_t xX = area_x(u.ra8, cF, u.rb8, nF);
-                        const uint32_t xY = area_x(u.rb8, cF, u.ra8, pF);
-                        UF[s] = leaf_decide<ONE>(live, UX[s], UY[s], xX, xY, qx, UU[s]);
-                    }
-                    bool uany = false;
-#pragma unroll
-                    for (uint32_t s = 0; s < WIN_UNIT; ++s) uany |= UU[s];
-                    if (__builtin_amdgcn_ballot_w64(uany)) {  // rare
-#pragma unroll
-                        for (uint32_t s = 0; s < WIN_UNIT; ++s) defer_leaf(P, tc, UU[s], kq, ub + s);
-                    }
-#pragma unroll
-                    for (uint32_t s = 0; s < WIN_UNIT; ++s)
-                        leaf_queue<ONE>(P, rq, tc, UF[s], UX[s], UY[s], kq, ub + s, Q.h2_w);
+                for (int p = 0; p < 8; ++p) {
+                    const uint32_t len = (p < 4 ? live0 : live1) ? re[p] - rs[p] : 0u;
+                    R.se[p] = rs[p] | (rs[p] + len) << 16;
+                    ucs[p] = p ? units : (o | rho << 6);
+                    items += len;
+                    units += (len + WIN_UNIT - 1) / WIN_UNIT;
                 }
-            };
-            if (one)
-                phase2(std::true_type{});
-            else
-                phase2(std::false_type{});
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 0; i < 4; ++i) R.uc[i] = ucs[2 * i] | ucs[2 * i + 1] << 16;
+                evald += items;
+                const uint32_t ucl = wave_incl_scan(units);
+                const uint32_t uex = ucl - units;  // first unit of this lane's virtual partner
+                const uint32_t nunits = __builtin_amdgcn_readlane(ucl, 63);
+                __builtin_amdgcn_wave_barrier();
+                // ---- phase 3: lane = unit of WIN_UNIT consecutive entries of one range
+                if (one)
+                    win_units<true>(P, tc, lf, part, vps, mark, uex, units, nunits, cnt);
+                else
+                    win_units<false>(P, tc, lf, part, vps, mark, uex, units, nunits, cnt);
+                __builtin_amdgcn_wave_barrier();
+            }
         });
-        if (one)
-            root_flush<true>(P, rq, tc);
-        else
-            root_flush<false>(P, rq, tc);
-        tile_done(P, S, cnt, rq.tested, leaves, pairs, evald, P.evaluated);
-        rq.tested = 0;
+        tile_done(P, S, cnt, 0, leaves, pairs, evald, P.evaluated);
     }
 }
 
@@ -1031,8 +1156,9 @@ __global__ __launch_bounds__(256) void k_exact(const CellParams P) {
         const DefItem it = P.def[i];
         const LeftDesc L = P.lefts[it.left];
         const PosDesc& S = P.pos[L.pos];
-        const uint32_t k = it.kflags & 0x3FFFFFFFu;
-        const bool conj = (it.kflags >> 30) & 1u, full = it.kflags >> 31;
+        const uint32_t k = it.kflags & DEF_K_MASK;
+        const bool conj = (it.kflags & DEF_CONJ) != 0, full = (it.kflags & DEF_FULL) != 0;
+        const uint32_t om = (it.kflags >> 27) & 3u;
         const uint64_t m = (uint64_t)L.q * k;
         ExactCtx e;
         make_exact_ctx(S.l, m, S.limit, e);
@@ -1042,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_exact(const CellParams P) {
         uint32_t c = exact_leaf(zre, zim, e, P.qr, [&](uint64_t r, u128 a4) {
             uint32_t slot = atomicAdd(P.nraw, 1u);
             if (slot < P.raw_cap) P.raw[slot] = HitOut{S.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64), L.pos, 0u};
-        });
+        }, full && om ? om : 3u);
         if (full && c) atomicAdd(P.cand + S.out, (unsigned long long)c);
     }
 }

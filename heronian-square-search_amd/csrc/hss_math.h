@@ -576,6 +576,203 @@ HSS_HD inline uint32_t window_ranges(const Window& w, const uint32_t* bt, uint32
     return merge4(s, e);
 }
 
+// ------------------------------------------------- area classes, orientation
+//
+// The class rule.  Every leaf is z = 2 F z' with F = fixed(l) fixed(m) (an
+// integer: the even powers of 2 and of the primes = 3 mod 4) and z' = u' v'
+// (or u' conj(v')) a product of Gaussian values pi^r conj(pi)^(4E - r) of
+// split primes p.  Such a value is 1 (mod 2) in its real part and 0 (mod 4) in
+// its imaginary part, and so is every product of them; hence Re z' is odd,
+// |Re z'| != |Im z'|, and Im z' = 0 only for the central-by-central leaf.
+//   mod 3: in Z[i]/3 = F_9 conjugation is the Frobenius, conj(pi) = pi^3, and
+//     pi^4 = p; a value is p^(3E) pi^(-2r): for p = 1 (mod 3) pi is a unit
+//     (+-1, +-i), for p = 2 (mod 3) pi^2 = +-i.  So a value is = +-1 (Im = 0
+//     mod 3) or = +-i (Re = 0 mod 3), the latter iff p = 2 (mod 3) and r is odd.
+//   mod 5: Z[i]/5 = F_5 x F_5 with conjugation swapping the factors; a value
+//     maps to (a^r b^(4E-r), b^r a^(4E-r)), a b = p, whose components differ by
+//     the factor (a/b)^(2(r-2E)) = p^(-2(r-2E)): equal (Im = 0 mod 5) unless
+//     p = +-2 (mod 5) and r is odd, then opposite (Re = 0 mod 5).
+// With value_class (hss_build.h) the class c = (c3, c5) of z' is the xor over
+// its factors (conjugating v keeps it), and exactly one of Re z', Im z' is
+// divisible by 3, Im iff c3 = 0 (by 5 when 5 does not divide l m, Im iff
+// c5 = 0).  The area tests of the reference (src/search_v18.cpp:441-453) then
+// reduce to classes and parities, for pairs whose area modulus h = (3 or 9) *
+// (1 or 5) (make_lane_ctx):
+//   orientation 1 (d = Re z, area4 = Im z, 2^3 | 2 F Im z' always):
+//     counts only if (c3 = 0 or 3 | l m) and (c5 = 0 or 5 | l m);
+//   orientation 2 (d = Im z, area4 = Re z = 2 F Re z', Re z' odd): counts only
+//     if 4 | F, i.e. l or m is even, and (c3 = 1 or 3 | l m) and (c5 = 1 or
+//     5 | l m).
+// (3 | l m puts 9 | F, so 9 divides both parts; 5 never divides F.)  The
+// constrained bits of a pair are AREA_C3 unless 3 | l m and AREA_C5 unless
+// 5 | l m; orientation 1 needs class 0 on them, orientation 2 class 3.
+static constexpr uint32_t AREA_C3 = 1, AREA_C5 = 2;
+
+HSS_HD inline uint32_t area_constrained(bool need9, bool need5) {
+    return (need9 ? 0u : AREA_C3) | (need5 ? AREA_C5 : 0u);
+}
+
+// The leaf angle theta = arg z modulo pi.  Orientation 1's range test depends
+// only on |cos theta| (|Re z| = 2 l^2 m^2 |cos theta|); orientation 2 of z is
+// orientation 1 of i z (theta + pi/2), whose class is c xor 3.  With
+// theta(u v) = theta_u + theta_v and theta(u conj v) = theta_u - theta_v, left
+// sets sorted by (area group, theta_u) give, per partner, index ranges that
+// hold exactly the leaves of one orientation and one class that can count.
+
+constexpr double PI_D = 3.141592653589793;
+
+// theta of (x, y) != (0, 0) modulo pi, in [0, pi): psi (the angle modulo pi/2
+// of psi_of) plus pi/2 in the second and fourth quadrants.  FP64: sort keys.
+HSS_HD inline double theta_of(double x, double y) {
+    const bool odd = (x <= 0 && y > 0) || (x >= 0 && y < 0);  // psi_of's second and fourth cases
+    const double t = psi_of(x, y) + (odd ? HALF_PI : 0.0);
+    return t < PI_D ? t : 0.0;
+}
+
+// The same from FP32 coordinates and atan2f (error < 5e-7 rad, as psi_of_f).
+HSS_HD inline double theta_of_f(double x, double y) {
+    const bool odd = (x <= 0 && y > 0) || (x >= 0 && y < 0);
+    const double t = psi_of_f(x, y) + (odd ? HALF_PI : 0.0);
+    return t < PI_D ? t : 0.0;
+}
+
+// Bucket of an angle in [0, pi] among nb buckets (nb a power of two).
+HSS_HD inline uint32_t theta_bucket(double t, uint32_t nb) {
+    const double b = t * ((double)nb * (1.0 / PI_D));
+    const uint32_t k = b > 0 ? (uint32_t)b : 0u;
+    return k < nb ? k : nb - 1;
+}
+
+// Group of an entry of class c in a left set whose sort key keeps the class
+// bits kmask (the bits any of its pairs constrains): the kept bits, packed.
+HSS_HD inline uint32_t area_group(uint32_t c, uint32_t kmask) {
+    return kmask == 3u ? c : (kmask == 1u ? (c & 1u) : (kmask == 2u ? (c >> 1) & 1u : 0u));
+}
+
+// Class of the j-th entry class allowed for a target class tc on the
+// constrained bits cm, the other kept bits (kmask & ~cm) taken from j.
+HSS_HD inline uint32_t allowed_class(uint32_t tc, uint32_t cm, uint32_t kmask, uint32_t j) {
+    const uint32_t free = kmask & ~cm;
+    uint32_t c = tc & cm;
+    if (free == 3u) c |= j & 3u;
+    else if (free) c |= (j & 1u) ? free : 0u;
+    return c;
+}
+
+// Number of entry classes allowed: 2^(free bits).
+HSS_HD inline uint32_t allowed_count(uint32_t cm, uint32_t kmask) {
+    const uint32_t free = kmask & ~cm;
+    return free == 3u ? 4u : (free ? 2u : 1u);
+}
+
+// Orientation-1 window of the pair in theta (mod pi): |cos theta| in S = [s0,
+// s1] (make_window's S), i.e. |theta - pi/2| in [asin s0, asin s1]: up to two
+// arcs [a, a + la], [b, b + lb] (cyclic; two arcs have la = lb), n = 1 or 2,
+// or n = -1 for all of it.
+// The widening is make_window's (the error budget there, with theta_u keys of
+// the same precision as psi keys and theta_v from psi_of_f plus a multiple of
+// pi/2).
+struct Window1 {
+    double a, la, b, lb;
+    int n;
+};
+
+// The arcs are kept at least WIN_GAP apart (two buckets of the coarsest group
+// slice, 256 buckets over pi: hss_types.h checks WIN_NB / 4 == 256), on the
+// circle and across its wrap point: a closer gap is filled (its leaves are
+// evaluated and count 0).  So the index ranges of the arcs' bucket-rounded
+// pieces never overlap and need no merging.
+constexpr double WIN_GAP = 2.0 * PI_D / 256.0;
+
+HSS_HD inline void make_window_o1(double lo, double hi, double tau, double R, Window1& w) {
+    const double inv = recip(R);
+    const bool low = lo <= tau;
+    double s0 = low ? 0.0 : (lo - tau) * inv;
+    double s1 = ((low ? (hi > -lo ? hi : -lo) : hi) + tau) * inv;
+    s0 -= 1e-6;
+    s1 += 1e-6;
+    s0 = s0 < 0.0 ? 0.0 : (s0 > 1.0 ? 1.0 : s0);
+    s1 = s1 < 0.0 ? 0.0 : (s1 > 1.0 ? 1.0 : s1);
+    const double a0 = (double)asinf((float)s0) - WIN_EPS, a1 = (double)asinf((float)s1) + WIN_EPS;
+    // gaps 2 a0 (between the arcs) and pi - 2 a1 (across pi/2 + pi/2)
+    const bool inner = 2.0 * a0 < WIN_GAP, outer = PI_D - 2.0 * a1 < WIN_GAP;
+    w.n = (inner && outer) ? -1 : ((inner || outer) ? 1 : 2);
+    // inner: [pi/2 - a1, pi/2 + a1]; outer: [pi/2 + a0, 3 pi/2 - a0]; else both halves
+    const double b0 = a0 > 0.0 ? a0 : 0.0;
+    w.a = inner ? HALF_PI - a1 : (outer ? HALF_PI + b0 : HALF_PI - a1);
+    w.la = inner ? 2.0 * a1 : (outer ? PI_D - 2.0 * b0 : a1 - a0);
+    w.b = HALF_PI + a0;
+    w.lb = a1 - a0;
+}
+
+HSS_HD inline double window_o1_fraction(const Window1& w) {
+    if (w.n < 0) return 1.0;
+    const double f = (w.la + (w.n == 2 ? w.lb : 0.0)) * (1.0 / PI_D);
+    return f > 1.0 ? 1.0 : f;
+}
+
+// Entries of one group (bucket table slice btg[0..nbg], absolute entry
+// indices; btg[0] and btg[nbg] bound the group) whose theta lies in the cyclic
+// arc [a + shift, a + shift + len] (len < pi): one or two index ranges.
+HSS_HD inline void arc_ranges(const uint32_t* btg, uint32_t nbg, double a, double len, double shift, uint32_t& s0,
+                              uint32_t& e0, uint32_t& s1, uint32_t& e1) {
+    constexpr double INV_PI = 0.31830988618379067;
+    double x = a + shift;
+    x -= PI_D * floor(x * INV_PI);
+    x = x < 0.0 ? 0.0 : (x >= PI_D ? 0.0 : x);
+    const double y = x + len;
+    const bool wrap = y >= PI_D;
+    const uint32_t t = btg[theta_bucket(wrap ? y - PI_D : y, nbg) + 1];
+    s0 = btg[theta_bucket(x, nbg)];
+    e0 = wrap ? btg[nbg] : t;
+    s1 = btg[0];
+    e1 = wrap ? t : btg[0];
+}
+
+// Up to four disjoint index ranges of one group for the leaves u * w, w the
+// partner (rotated by i for orientation 2) of angle theta_w, or u * conj(w)
+// (conj): theta_u in W - theta_w, or W + theta_w.  The arcs and their wrapped
+// pieces are at least WIN_GAP (two buckets) apart, so the bucket-rounded
+// ranges are disjoint.  Returns the entries covered.
+HSS_HD inline uint32_t window_o1_ranges(const Window1& w, const uint32_t* btg, uint32_t nbg, double theta_w, bool conj,
+                                        uint32_t* s, uint32_t* e) {
+    const double shift = conj ? theta_w : -theta_w;
+    uint32_t s0, e0, s1, e1, s2, e2, s3, e3;
+    arc_ranges(btg, nbg, w.a, w.la, shift, s0, e0, s1, e1);
+    arc_ranges(btg, nbg, w.b, w.lb, shift, s2, e2, s3, e3);
+    const bool all = w.n < 0, two = w.n == 2;
+    s[0] = all ? btg[0] : s0;
+    e[0] = all ? btg[nbg] : e0;
+    s[1] = all ? btg[0] : s1;
+    e[1] = all ? btg[0] : e1;
+    s[2] = two ? s2 : 0u;
+    e[2] = two ? e2 : 0u;
+    s[3] = two ? s3 : 0u;
+    e[3] = two ? e3 : 0u;
+    return (e[0] - s[0]) + (e[1] - s[1]) + (e[2] - s[2]) + (e[3] - s[3]);
+}
+
+// Band of one window record: the pair's term-1 distance band (widened; the
+// narrowed bounds are w - tau), term-2 bound, l^4 + m^4.
+struct Band {
+    double mid, half_w, h2_w, tau, lm;
+};
+
+// Orientation-1 terms of a leaf with X = Re(u w) (the class rule already
+// holds): bit 0 term 1 (d = +|2X|), bit 1 term 2 (d = -|2X|, not on ONE tiles);
+// und when the FP64 band leaves the leaf undecided (then no term is returned).
+template <bool ONE>
+HSS_HD inline uint32_t o1_decide(bool live, double X, const Band& b, bool& und) {
+    const double ax = X < 0 ? -X : X;
+    const double dx = ax - b.mid;
+    const double adx = dx < 0 ? -dx : dx;
+    const bool t1w = adx <= b.half_w, t1n = adx <= b.half_w - b.tau;
+    const bool t2w = !ONE && ax <= b.h2_w, t2n = !ONE && ax <= b.h2_w - b.tau;
+    const bool u = live & ((t1w != t1n) | (t2w != t2n));
+    und = u;
+    return ((live & t1n & !u) ? 1u : 0u) | ((live & t2n & !u) ? 2u : 0u);
+}
+
 // ------------------------------------------------------------ exact stage
 
 struct ExactCtx {
@@ -686,13 +883,16 @@ struct HitOut {
     uint32_t pad;
 };
 
-// Full reference filter chain of one leaf z (exact).  Returns the candidates;
-// every exact fourth power is passed to sink(n, |area4|).
+// Full reference filter chain of one leaf z (exact).  Returns the candidates
+// of the orientations in omask (bit 0: d = Re z, bit 1: d = Im z); every exact
+// fourth power of those orientations is passed to sink(n, |area4|).
 template <typename Sink>
-HSS_HD inline uint32_t exact_leaf(i128 zre, i128 zim, const ExactCtx& e, const QR4& q, Sink&& sink) {
+HSS_HD inline uint32_t exact_leaf(i128 zre, i128 zim, const ExactCtx& e, const QR4& q, Sink&& sink,
+                                  uint32_t omask = 3u) {
     uint32_t cand = 0;
     for (int o = 0; o < 2; ++o) {
         if (o == 1 && (zre == zim || zre == -zim)) break;
+        if (!((omask >> o) & 1u)) continue;
         i128 d = o == 0 ? zre : zim;
         i128 a4 = o == 0 ? zim : zre;
         if (a4 == 0 || ((uint64_t)a4 & 7u)) continue;

@@ -281,7 +281,7 @@ bool plan_lefts(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
                 L.off = (uint32_t)C.entries;  // relative to the chunk until merged
                 L.H = (uint32_t)H;
                 L.q = (uint32_t)q;
-                L.trivial = f.ns == 0 ? 1u : 0u;
+                L.flags = left_flags(f.ns == 0, l);
                 L.q3 = (uint32_t)(q % 3);
                 L.q5 = (uint32_t)(q % 5);
                 C.entries += L.H;

@@ -13,6 +13,7 @@ static constexpr int MAX_INERT = 8;  // distinct primes = 3 (mod 4)
 // sorted by the angle of their entries modulo pi/2 (the central entry stays
 // last) with a table of WIN_NB + 1 bucket starts.
 static constexpr uint32_t WIN_NB = 1024;
+static_assert(WIN_NB / 4 == 256, "WIN_GAP (hss_math.h) is two buckets of the coarsest group slice");
 static constexpr uint32_t WIN_MAX_SORT = 8192;
 static constexpr uint32_t WIN_MIN_SORT = 8;
 // Window-mode leaves are evaluated in units of WIN_UNIT consecutive sorted
@@ -20,6 +21,8 @@ static constexpr uint32_t WIN_MIN_SORT = 8;
 // its left set, so the left-entry arrays carry WIN_UNIT entries of padding.
 static constexpr uint32_t WIN_UNIT = 4;
 static constexpr uint32_t NO_TABLE = 0xFFFFFFFFu;
+// cache residue word: re mod 180 (bits 0-7), area class (bits 8-9), im mod 180 << 16
+static constexpr uint32_t CRES_CLASS_SHIFT = 8;
 // Leaves queued for the root test are named by (entry u, partner entry within
 // its half-set, k, conjugate, counted terms) packed into 64 bits: u below
 // 2^17, the partner entry below 2^15 (the planner and the table build check),
@@ -53,10 +56,19 @@ struct LeftDesc {
     uint32_t off;      // first entry in the left arrays
     uint32_t H;        // entries; entry H-1 is the real (central) one
     uint32_t q;        // product of the shared prime powers: m = q * k
-    uint32_t trivial;  // l has no split prime: partners without one are skipped
+    uint32_t flags;    // LEFT_TRIVIAL | area key mask << 1 (left_kmask)
     uint32_t q3, q5;   // q mod 3, q mod 5
     uint32_t bt;       // bucket table index of a sorted left set, else NO_TABLE
 };
+
+// LeftDesc::flags: l has no split prime (partners without one are skipped);
+// bits 1-2: the class bits any pair of the left set constrains (AREA_C3 when
+// 3 does not divide l, AREA_C5 when 5 does not), the group key of a sorted set.
+static constexpr uint32_t LEFT_TRIVIAL = 1;
+HSS_HD inline uint32_t left_kmask(const LeftDesc& L) { return (L.flags >> 1) & 3u; }
+HSS_HD inline uint32_t left_flags(bool trivial, uint64_t l) {
+    return (trivial ? LEFT_TRIVIAL : 0u) | ((l % 3 ? AREA_C3 : 0u) | (l % 5 ? AREA_C5 : 0u)) << 1;
+}
 
 // Build recipe of one left set: fixed part and merged exponents of l's split primes.
 struct LeftBuild {
@@ -87,8 +99,13 @@ struct BatchHeader {
     uint32_t pad[3];
 };
 
-// A leaf sent to the exact kernel.  kflags = k | conj << 30 | full << 31, where
-// full = 1 means the fast path left the leaf's count undecided.
+// A leaf sent to the exact kernel.  kflags = k | omask << 27 | conj << 30 |
+// full << 31, where full = 1 means the fast path left the leaf's count
+// undecided and the exact kernel counts the terms of the orientations in omask
+// (bit 0: orientation 1, bit 1: orientation 2; 0 means both).
+static constexpr uint32_t DEF_K_MASK = (1u << 27) - 1;
+static constexpr uint32_t DEF_O1 = 1u << 27, DEF_O2 = 2u << 27, DEF_CONJ = 1u << 30, DEF_FULL = 1u << 31;
+
 struct DefItem {
     uint32_t left;
     uint32_t ui;    // left entry index within the left set

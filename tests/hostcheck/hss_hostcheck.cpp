@@ -85,6 +85,7 @@ void build(HostTables& T, uint64_t n) {
         for (uint32_t e = T.off[i]; e < T.off[i + 1]; ++e) {
             uint32_t dig[MAX_SPLIT];
             decode_rank(e - T.off[i], k, es, dig);
+            const uint32_t cls = digits_class(k, ps, dig);
             int64_t re = fixed, im = 0;
             for (uint32_t j = 0; j < k; ++j) {
                 GInt v = gauss_value(T.pa[ps[j]], T.pb[ps[j]], es[j], dig[j]);
@@ -96,7 +97,8 @@ void build(HostTables& T, uint64_t n) {
             T.cre[e] = (double)re;
             T.cim[e] = (double)im;
             int64_t r1 = re % (int64_t)QA, i1 = im % (int64_t)QA;
-            T.cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
+            T.cres[e] = (uint32_t)(r1 < 0 ? r1 + QA : r1) | cls << CRES_CLASS_SHIFT |
+                        (uint32_t)(i1 < 0 ? i1 + QA : i1) << 16;
         }
     }
     uint32_t root = 2;
@@ -121,6 +123,7 @@ typedef struct {
     uint64_t cells;          // leaves evaluated
     uint64_t window_tiles;
     uint64_t window_items;   // leaves evaluated in window-mode tiles
+    uint64_t class_mismatch; // leaves whose class-rule area decision differs from the residue test
 } hc_stats;
 
 static Tuning g_tune;
@@ -130,6 +133,14 @@ static Tuning g_tune;
 void hc_set_tuning(double frac, uint32_t first_class, uint32_t tiles_per_wave) {
     for (int i = 0; i < WEIGHT_CLASSES; ++i) g_tune.win_max_frac[i] = (uint32_t)i >= first_class ? frac : 0.0;
     g_tune.tiles_per_wave = tiles_per_wave;
+}
+
+// The planner's default tuning (the device context's defaults), or one window
+// threshold per weight class (WEIGHT_CLASSES values).
+void hc_set_tuning_classes(const double* fracs, uint32_t tiles_per_wave) {
+    const Tuning d;
+    for (int i = 0; i < WEIGHT_CLASSES; ++i) g_tune.win_max_frac[i] = fracs ? fracs[i] : d.win_max_frac[i];
+    g_tune.tiles_per_wave = tiles_per_wave ? tiles_per_wave : d.tiles_per_wave;
 }
 
 // Search root positions (roots[s] at limits[s]) with the kernels' arithmetic,
@@ -149,32 +160,40 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
     // left sets; sorted ones (window mode) ordered by psi with the central entry last
     std::vector<i128> lre(P.left_entries), lim(P.left_entries);
     std::vector<LeftFast> lfast(P.left_entries);
+    std::vector<uint32_t> lcls(P.left_entries);  // class of each stored left entry (checks only)
     std::vector<uint32_t> btab(P.sorted.size() * (WIN_NB + 1));
     for (size_t li = 0; li < P.lefts.size(); ++li) {
         const LeftBuild& b = P.builds[li];
         const LeftDesc& L = P.lefts[li];
         std::vector<GInt> val(L.H);
+        std::vector<uint32_t> cls(L.H);
         for (uint32_t t = 0; t < L.H; ++t) {
             uint32_t dig[MAX_SPLIT];
             decode_rank(t, b.k, b.E, dig);
             GInt z = {(i128)b.fixed, 0};
             for (uint32_t j = 0; j < b.k; ++j) z = gmul(z, gauss_value(T.pa[b.p[j]], T.pb[b.p[j]], b.E[j], dig[j]));
             val[t] = z;
+            cls[t] = digits_class(b.k, b.p, dig);
         }
         std::vector<uint32_t> perm(L.H);
         for (uint32_t t = 0; t < L.H; ++t) perm[t] = t;
         if (L.bt != NO_TABLE) {
-            const uint32_t n = L.H - 1;
-            std::vector<float> key(n);  // FP32 keys, as the device sort stores them
-            for (uint32_t t = 0; t < n; ++t)
-                key[t] = (float)psi_of(i128_to_double(val[t].re), i128_to_double(val[t].im));
+            // k_left_sort: (area group, theta) keys, FP32 angles, group slices of the buckets
+            const uint32_t n = L.H - 1, kmask = left_kmask(L), gbits = __builtin_popcount(kmask);
+            const uint32_t nbg = WIN_NB >> gbits;
+            std::vector<float> key(n);
+            std::vector<uint32_t> bucket(n);
+            for (uint32_t t = 0; t < n; ++t) {
+                key[t] = (float)theta_of(i128_to_double(val[t].re), i128_to_double(val[t].im));
+                bucket[t] = area_group(cls[t], kmask) * nbg + theta_bucket((double)key[t], nbg);
+            }
             std::sort(perm.begin(), perm.begin() + n, [&](uint32_t x, uint32_t y) {
-                return key[x] < key[y] || (key[x] == key[y] && x < y);
+                return std::tie(bucket[x], key[x], x) < std::tie(bucket[y], key[y], y);
             });
             uint32_t* bt = btab.data() + (size_t)L.bt * (WIN_NB + 1);
             for (uint32_t bk = 0; bk <= WIN_NB; ++bk) {
                 uint32_t c = 0;
-                while (c < n && psi_bucket<WIN_NB>(key[perm[c]]) < bk) ++c;
+                while (c < n && bucket[perm[c]] < bk) ++c;
                 bt[bk] = bk == WIN_NB ? n : c;
             }
         }
@@ -188,6 +207,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             f.ra8 = smod128(z.re, QA) << 8;
             f.rb8 = smod128(z.im, QA) << 8;
             lfast[L.off + t] = f;
+            lcls[L.off + t] = cls[perm[t]];
         }
     }
     std::fill(cand, cand + n_roots, 0);
@@ -195,7 +215,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
     std::fill(pairs, pairs + n_roots, 0);
     hc_stats S{};
     struct Def {
-        uint32_t left, ui, vidx, k, conj, full;
+        uint32_t left, ui, vidx, k, conj, full, omask;
     };
     std::vector<Def> defs;
     // one leaf: u (entry ui of left set li) times v or conj(v) (cj)
@@ -203,7 +223,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                     uint32_t k, uint32_t cj, bool both_real) {
         const LeftFast& u = lfast[L.off + ui];
         const double c = T.cre[vidx], d = T.cim[vidx];
-        const uint32_t c180 = T.cres[vidx] & 0xFFFFu, d180 = T.cres[vidx] >> 16;
+        const uint32_t c180 = T.cres[vidx] & 0xFFu, d180 = T.cres[vidx] >> 16;
         const uint32_t ua = u.ra8 >> 8, ub = u.rb8 >> 8;
         const double t1 = u.im * d, t2 = u.re * d;
         const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
@@ -226,15 +246,26 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             eval_real_leaf(vX, ctx, r);
         else
             eval_leaf(X, Y, vX, vY, ctx, r);
+        // the class rule against the residue test of both orientations (area 0 aside)
+        {
+            const uint64_t mm = (uint64_t)L.q * k;
+            const uint32_t cm = area_constrained(D.l3 == 0 || mm % 3 == 0, D.l5 != 0 && mm % 5 != 0);
+            const uint32_t c = (lcls[L.off + ui] ^ ((T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u)) & cm;
+            const bool even = D.l_even || !(k & 1u);
+            const bool rule1 = c == 0u, rule2 = even && c == cm;
+            const bool res1 = area_ok(vY, ctx), res2 = area_ok(vX, ctx);
+            // orientation 1 of a real leaf has area 0 (the residue test sees 0 as divisible)
+            if ((!both_real && rule1 != res1) || rule2 != res2) ++S.class_mismatch;
+        }
         if (r.und) {
-            defs.push_back({li, ui, vidx, k, cj, 1});
+            defs.push_back({li, ui, vidx, k, cj, 1, 3});
             ++S.deferred_full;
             return;
         }
         cand[D.out] += r.cnt;
         for (int tm = 0; tm < 4; ++tm)
             if (r.f[tm] && root_near_integer(r.zz[tm])) {
-                defs.push_back({li, ui, vidx, k, cj, 0});
+                defs.push_back({li, ui, vidx, k, cj, 0, 3});
                 ++S.deferred_root;
             }
     };
@@ -250,7 +281,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                 for (uint32_t j = 0; j < D.ninert; ++j) ok = ok && !(k * D.inert_inv[j] <= D.inert_thr[j]);
                 if (!ok) continue;
                 uint32_t Hk = T.off[k + 1] - T.off[k];
-                if (L.trivial && Hk == 1) continue;
+                if ((L.flags & LEFT_TRIVIAL) && Hk == 1) continue;
                 pairs[D.out] += 1;
                 uint64_t m = (uint64_t)L.q * k;
                 LaneCtx ctx;
@@ -260,26 +291,95 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                     const bool vc = jj + 1 == Hk;
                     leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
                     // the central u: u*v only (real-leaf rule when v is real too)
-                    leaf(Tl.left, L, D, ctx, L.H - 1, vidx, k, 0, vc);
+                    if (mode == 0) leaf(Tl.left, L, D, ctx, L.H - 1, vidx, k, 0, vc);
                     if (mode == 0) {
                         for (uint32_t ui = 0; ui + 1 < L.H; ++ui)
                             for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) leaf(Tl.left, L, D, ctx, ui, vidx, k, cj, false);
                         continue;
                     }
+                    // cell_win_kernel: passes over (class variant j, orientation rho)
                     const double m2 = (double)(m * m);
-                    Window w;
-                    make_window((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, ctx.tau, 2.0 * D.l2_d * m2, w);
-                    const double pv = psi_of_f(T.cre[vidx], T.cim[vidx]);
+                    Window1 w;
+                    make_window_o1((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, ctx.tau, 2.0 * D.l2_d * m2, w);
+                    const double c = T.cre[vidx], d = T.cim[vidx];
+                    const double thv = theta_of_f(c, d);
+                    const uint32_t vcls = (T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u;
+                    const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
+                    const uint32_t cm = area_constrained(need9, need5), kmask = left_kmask(L);
+                    const uint32_t nbg = WIN_NB >> __builtin_popcount(kmask);
+                    const bool even = D.l_even || !(k & 1u);
+                    const bool one = tile_one_term(L.q, Tl.k0, D.l4_d, D.l2_d, D.n4_d);
+                    const Band bd{ctx.mid, ctx.half_w, ctx.h2_w, ctx.tau, ctx.lm};
                     const uint32_t* bt = btab.data() + (size_t)L.bt * (WIN_NB + 1);
-                    for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
-                        uint32_t rs[4], re[4];
-                        window_ranges<WIN_NB>(w, bt, L.H - 1, pv, cj != 0, rs, re);
-                        for (int p = 0; p < 4; ++p)
-                            for (uint32_t ui = rs[p]; ui < re[p]; ++ui) {
-                                ++S.window_items;
-                                leaf(Tl.left, L, D, ctx, ui, vidx, k, cj, false);
+                    {
+                        // central_win: the central u by the class rule
+                        ++S.cells;
+                        const double cu = lfast[L.off + L.H - 1].re;
+                        const double X = cu * c, Y = cu * d;
+                        const bool ok1 = !vc && (vcls & cm) == 0u, ok2 = even && ((vcls ^ 3u) & cm) == 0u;
+                        const bool real2 = ok2 && vc;
+                        const double h2n = bd.h2_w - bd.tau;
+                        bool u1, u2;
+                        const uint32_t f1 = one ? o1_decide<true>(ok1, X, bd, u1) : o1_decide<false>(ok1, X, bd, u1);
+                        const uint32_t f2 = one ? o1_decide<true>(ok2 && !vc, Y, bd, u2)
+                                                : o1_decide<false>(ok2 && !vc, Y, bd, u2);
+                        const bool und = u1 || u2 || (real2 && bd.h2_w >= 0.0 && h2n < 0.0);
+                        if (und) {
+                            defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 1, 3});
+                            ++S.deferred_full;
+                        } else {
+                            const uint32_t f = f1 | f2 << 2;
+                            cand[D.out] += __builtin_popcount(f) + ((real2 && h2n >= 0.0) ? 1u : 0u);
+                            const double ax = std::fabs(X), ay = std::fabs(Y);
+                            bool pass = false;
+                            for (uint32_t t = 0; t < 4; ++t)
+                                pass |= ((f >> t) & 1u) &&
+                                        root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, t < 2u ? ax : ay, bd.lm));
+                            if (pass) {
+                                defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 0, 3});
+                                ++S.deferred_root;
                             }
+                        }
                     }
+                    for (uint32_t j = 0; j < allowed_count(cm, kmask); ++j)
+                        for (uint32_t rho = 0; rho < (even ? 2u : 1u); ++rho) {
+                            const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), cm, kmask, j), kmask);
+                            double thw = thv + (rho ? HALF_PI : 0.0);
+                            thw = thw >= PI_D ? thw - PI_D : thw;
+                            const double ce = rho ? -d : c, de0 = rho ? c : d;
+                            for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
+                                uint32_t rs[4], re[4];
+                                window_o1_ranges(w, bt + g * nbg, nbg, thw, cj != 0, rs, re);
+                                const double de = cj ? -de0 : de0;
+                                for (int p = 0; p < 4; ++p)
+                                    for (uint32_t ui = rs[p]; ui < re[p]; ++ui) {
+                                        ++S.window_items;
+                                        ++S.cells;
+                                        const LeftFast& u = lfast[L.off + ui];
+                                        const double X = __builtin_fma(u.re, ce, -(u.im * de));
+                                        bool und;
+                                        const uint32_t f = one ? o1_decide<true>(true, X, bd, und)
+                                                               : o1_decide<false>(true, X, bd, und);
+                                        // every leaf in a range passes its orientation's area test
+                                        {
+                                            const uint32_t cr = (lcls[L.off + ui] ^ vcls) & cm;
+                                            if (cr != (rho ? cm : 0u)) ++S.class_mismatch;
+                                        }
+                                        if (und) {
+                                            defs.push_back({Tl.left, ui, vidx, k, cj, 1, rho ? 2u : 1u});
+                                            ++S.deferred_full;
+                                            continue;
+                                        }
+                                        cand[D.out] += __builtin_popcount(f);
+                                        const double ax = std::fabs(X);
+                                        if (((f & 1u) && root_near_integer(__builtin_fma(-2.0, ax, bd.lm))) ||
+                                            ((f & 2u) && root_near_integer(__builtin_fma(2.0, ax, bd.lm)))) {
+                                            defs.push_back({Tl.left, ui, vidx, k, cj, 0, 3});
+                                            ++S.deferred_root;
+                                        }
+                                    }
+                            }
+                        }
                 }
             }
         }
@@ -301,7 +401,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             ++S.raw_roots;
             if (accept_hit(D.l, m, r, D.limit, a4) && seen.insert({D.l, m, r}).second)
                 got.push_back(hc_hit{D.l, m, r, (uint64_t)a4, (uint64_t)(a4 >> 64)});
-        });
+        }, df.full ? df.omask : 3u);
         if (df.full) cand[D.out] += c;
     }
     std::sort(got.begin(), got.end(),

@@ -23,7 +23,7 @@ class HcHit(ctypes.Structure):
 class HcStats(ctypes.Structure):
     _fields_ = [("deferred_full", ctypes.c_uint64), ("deferred_root", ctypes.c_uint64),
                 ("raw_roots", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("window_tiles", ctypes.c_uint64),
-                ("window_items", ctypes.c_uint64)]
+                ("window_items", ctypes.c_uint64), ("class_mismatch", ctypes.c_uint64)]
 
 
 DEFAULT_TUNING = (0.5, 3, 8)  # threshold for weight classes >= 3 (H >= 64), tiles_per_wave (hss_plan.h Tuning)
@@ -65,6 +65,8 @@ def hc_search(hc, nmax, limit, roots):
     rc = hc.hc_search(ctypes.c_uint64(nmax), ctypes.c_uint64(limit), R, n, c, lv, pr, H, 64, ctypes.byref(nh),
                       ctypes.byref(st))
     assert rc == 0
+    # the class rule (hss_math.h) decides every evaluated leaf's area tests as the residues do
+    assert st.class_mismatch == 0
     hits = [(h.l, h.m, h.n, ((h.hi << 64) | h.lo) // 4) for h in H[: nh.value]]
     return list(c), list(lv), list(pr), hits, st
 
