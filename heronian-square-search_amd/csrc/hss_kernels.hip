@@ -240,6 +240,13 @@ __device__ inline GInt left_value(const LeftBuild& b, uint32_t t, const uint32_t
     return z;
 }
 
+// Class of the entry of rank t of a left set (the class rule of hss_math.h).
+__device__ inline uint32_t left_class(const LeftBuild& b, uint32_t t) {
+    uint32_t dig[MAX_SPLIT];
+    decode_rank(t, b.k, b.E, dig);
+    return digits_class(b.k, b.p, dig);
+}
+
 // One block per left set (grid-stride over hdr->nleft).  Entry t0 holds rank
 // perm[off + t0] for a sorted left set (window mode), rank t0 otherwise.
 __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, const LeftBuild* builds,
@@ -259,18 +266,11 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.ra8 = smod128(z.re, QA) << 8;
-            f.rb8 = smod128(z.im, QA) << 8;
+            f.cls = left_class(b, rank);
+            f.pad = 0;
             lfast[e] = f;
         }
     }
-}
-
-// Class of the entry of rank t of a left set (the class rule of hss_math.h).
-__device__ inline uint32_t left_class(const LeftBuild& b, uint32_t t) {
-    uint32_t dig[MAX_SPLIT];
-    decode_rank(t, b.k, b.E, dig);
-    return digits_class(b.k, b.p, dig);
 }
 
 // Bucket of a sorted left set's entry: its area group's slice of the WIN_NB
@@ -460,12 +460,9 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
 // test (leaves the FP64 band left undecided go straight to the deferred
 // buffer, defer_leaf).  Entry: X, Y of z/2 and the packed id: (u | partner
 // entry << 17, k | conj << 27 | counted terms << 28), the terms as bits 0-3 for
-// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).
-// In the ZZ form (window tiles with one term per orientation: bits 0 and 2
-// only) the entry holds the zz of its lowest term (lm - 2|X| with bit 0, else
-// lm - 2|Y|) in place of X and lm - 2|Y| in place of Y, so a round needs no
-// lm and no select.  Slots head .. head + n - 1 (mod QCAP) are live.  One
-// 24-byte entry per slot: one address per push or pop.
+// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).  Slots
+// head .. head + n - 1 (mod QCAP) are live.  One 24-byte entry per slot: one
+// address per push or pop.
 struct QEntry {
     double x, y;
     uint32_t lo, hi;
@@ -493,7 +490,7 @@ __device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj <<
 // buffer) and the leaf is queued again while terms remain, so every round
 // tests a full wave of terms.  FULL: the queue holds at least 64 entries
 // (every lane has one; the lane masks fold away).
-template <bool ZZ, bool FULL>
+template <bool FULL>
 __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     const uint32_t lane = __lane_id();
     const uint32_t count = FULL ? 64u : (q.n < 64u ? q.n : 64u);
@@ -510,28 +507,22 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     q.tested += count;
     const uint32_t k = qe.hi & 0x7FFFFFFu;
     const uint32_t terms = in ? qe.hi >> 28 : 0u;  // every queued entry has a counted term
-    double zz;
-    if constexpr (ZZ) {
-        zz = qe.x;  // terms are 1, 4 or 5
-    } else {
-        const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
-        const double lm = lm_of(tc.q, k, tc.l4_d);
-        const double a = 2.0 * __builtin_fabs(t < 2u ? qe.x : qe.y);
-        zz = (t & 1u) ? lm + a : lm - a;
-    }
+    const uint32_t t = (uint32_t)__builtin_ctz(terms | 16u);  // 4 on lanes without an entry
+    const double lm = lm_of(tc.q, k, tc.l4_d);
+    const double a = 2.0 * __builtin_fabs(t < 2u ? qe.x : qe.y);
+    const double zz = (t & 1u) ? lm + a : lm - a;
     const bool pass = in & root_near_integer(zz);
     if (__builtin_amdgcn_ballot_w64(pass)) {
         const uint32_t ui = qe.lo & MAX_LEFT_ENTRIES, jj = qe.lo >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
         push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((qe.hi >> 27) & 1u) << 30});
     }
-    // terms left after the lowest (ZZ: terms are 1, 4 or 5)
-    const uint32_t rest = ZZ ? (terms == 5u ? 4u : 0u) : terms & (terms - 1u);
+    const uint32_t rest = terms & (terms - 1u);  // terms left after the lowest
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
     if (again) {
         if (rest != 0u)
             lds_at(q.e, lane_rank(again, q.head + q.n) & (uint32_t)(QCAP - 1)) =
-                QEntry{ZZ ? qe.y : qe.x, qe.y, qe.lo, (qe.hi & 0x0FFFFFFFu) | rest << 28};
+                QEntry{qe.x, qe.y, qe.lo, (qe.hi & 0x0FFFFFFFu) | rest << 28};
         __builtin_amdgcn_wave_barrier();
         q.n += (uint32_t)__popcll(again);
     }
@@ -539,78 +530,43 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
 
 // Append the lanes with pred set to the queue; full waves of entries are
 // drained (the queue never holds more than 127 live entries).
-template <bool ZZ>
 __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCtx& tc, bool pred, double X, double Y,
                                  uint32_t idlo, uint32_t idhi) {
     const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
     if (!mask) return;
     if (pred) lds_at(q.e, lane_rank(mask, q.head + q.n) & (uint32_t)(QCAP - 1)) = QEntry{X, Y, idlo, idhi};
     q.n += (uint32_t)__popcll(mask);
-    while (q.n >= 64) root_round<ZZ, true>(P, q, tc);
+    while (q.n >= 64) root_round<true>(P, q, tc);
 }
 
-template <bool ZZ>
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
-    while (q.n) root_round<ZZ, false>(P, q, tc);
+    while (q.n) root_round<false>(P, q, tc);
     q.head = 0;  // empty: restart at slot 0 (full rounds rely on head = 0 or 64)
 }
 
-// One leaf z/2 = X + iY of the fast path: the decisions of eval_leaf (hss_math.h)
-// for lane data, with the area residues of X and Y given as the area test's
-// products xX, xY (area_x).  Returns the
-// leaf's counted terms (bits 0-3: term1, term2 of orientation 1, then of
-// orientation 2), which are its decided candidates (counted as the root queue
-// tests them); undecided is set (and no term returned) when the FP64 band
-// leaves the leaf to the exact stage.  ONE:
-// the pair has h2_w < 0 (tile_one_term), so term2 of
-// either orientation is false and its compares are skipped (same result).
-template <bool ONE = false>
-__device__ inline uint32_t leaf_decide(bool live, double X, double Y, uint32_t xX, uint32_t xY, const LaneCtx& c,
-                                       bool& undecided) {
-    // bitwise logic on lane masks: no short-circuit branches
-    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
-    const bool a1 = live & area_ok_x(xY, c);
-    const bool a2 = live & area_ok_x(xX, c);
-    // orientation 1 (d = X) and 2 (d = Y): narrowed ranges decide, the widened
-    // ones minus the narrowed ones are the undecided band
-    const double dx = __builtin_fabs(ax - c.mid), dy = __builtin_fabs(ay - c.mid);
-    const bool t1w = dx <= c.half_w, t1n = dx <= c.half_n;
-    const bool t2w = !ONE && ax <= c.h2_w, t2n = !ONE && ax <= c.h2_n;
-    const bool t3w = dy <= c.half_w, t3n = dy <= c.half_n;
-    const bool t4w = !ONE && ay <= c.h2_w, t4n = !ONE && ay <= c.h2_n;
-    const double dd = ax - ay;
-    const bool eq = __builtin_fabs(dd) <= c.tau;
-    // |Y| <= tau (|X| <= tau) leaves the area of orientation 1 (2) possibly 0.
-    // With ONE that can only matter where the orientation's own range fails:
-    // then l^4 >= lo > 2 tau, and t1w gives |X| <= (l^4 + tau)/2 + tau/16 <=
-    // 0.79 l^4, so |Y| = sqrt(l^4 m^4 - X^2) >= 0.6 l^4 > tau (m >= l); the
-    // same for t3w and |X|.  Those compares are dropped there.
-    // (a narrowed range lies inside its widened one: "w and not n" is w != n)
-    const bool und = ONE ? (a1 & (t1w != t1n)) | (a2 & ((t3w != t3n) | eq))
-                         : (a1 & ((t1w != t1n) | (t2w != t2n) | (ay <= c.tau))) |
-                               (a2 & ((t3w != t3n) | (t4w != t4n) | eq | (ax <= c.tau)));
-    const bool ok = !und;
-    const bool o2 = a2 & !eq & ok;
-    const bool f0 = a1 & t1n & ok, f1 = a1 & t2n & ok, f2 = o2 & t3n, f3 = o2 & t4n;
-    const uint32_t terms = (f0 ? 1u : 0u) | (f1 ? 2u : 0u) | (f2 ? 4u : 0u) | (f3 ? 8u : 0u);
-    undecided = und;
-    return terms;
+// One leaf z/2 = X + iY of the plain kernel by the class rule (hss_math.h):
+// ok1 / ok2 say whether orientation 1 (d = 2X) / 2 (d = 2Y) passes the area
+// tests, from the classes of u and v and the pair; the FP64 band of each
+// allowed orientation decides its terms.  Returns the counted terms (bits 0-1:
+// term 1, 2 of orientation 1 on |X|; bits 2-3: of orientation 2 on |Y|), the
+// leaf's decided candidates (counted as the root queue tests them); und is set
+// (and no term returned) when a band leaves the leaf to the exact stage.  ONE:
+// the pair has h2_w < 0 (tile_one_term), so term 2 of either orientation is
+// false and its compares are skipped (same result).
+template <bool ONE>
+__device__ inline uint32_t leaf_decide(bool ok1, bool ok2, double X, double Y, const Band& b, bool& und) {
+    bool u1, u2;
+    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
+    const uint32_t f2 = o1_decide<ONE>(ok2, Y, b, u2);
+    und = u1 | u2;
+    return und ? 0u : (f1 | f2 << 2);
 }
 
 // Queue a decided leaf with counted terms (from leaf_decide; kq = k | conj <<
-// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test; ZZ: the
-// queue form with the lowest term's zz and lm - 2|Y| (one rounding each, as in
-// the rounds).
-template <bool ZZ = false>
+// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test.
 __device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t terms, double X,
-                                  double Y, uint32_t kq, uint32_t idlo, double lm = 0.0) {
-    if constexpr (ZZ) {
-        const double zy = __builtin_fma(-2.0, __builtin_fabs(Y), lm);
-        root_push<true>(P, rq, tc, terms != 0u, (terms & 1u) ? __builtin_fma(-2.0, __builtin_fabs(X), lm) : zy, zy,
-                        idlo, kq | terms << 28);
-    }
-    else
-        root_push<false>(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
+                                  double Y, uint32_t kq, uint32_t idlo) {
+    root_push(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
 }
 
 // An undecided leaf goes whole to the exact stage (k_exact counts all its
@@ -619,57 +575,41 @@ __device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const Tile
 __device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool und, uint32_t kq, uint32_t idlo) {
     const uint32_t k = kq & 0x7FFFFFFu;
     const uint32_t vidx = und ? P.off[k] + (idlo >> ID_U_BITS) : 0u;
-    push_global(P, und, DefItem{tc.left, idlo & MAX_LEFT_ENTRIES, vidx, k | ((kq >> 27) & 1u) << 30 | 0x80000000u});
+    push_global(P, und, DefItem{tc.left, idlo & MAX_LEFT_ENTRIES, vidx, k | ((kq >> 27) & 1u) << 30 | DEF_FULL});
 }
 
-template <bool ZZ = false>
-__device__ inline void leaf_fast(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool live, double X, double Y,
-                                 uint32_t xX, uint32_t xY, const LaneCtx& c, uint32_t kq, uint32_t idlo) {
-    bool und;
-    const uint32_t terms = leaf_decide(live, X, Y, xX, xY, c, und);
-    if (__builtin_amdgcn_ballot_w64(und)) defer_leaf(P, tc, und, kq, idlo);
-    leaf_queue<ZZ>(P, rq, tc, terms, X, Y, kq, idlo, c.lm);
-}
-
-// The per-pair context and residues of one partner entry v (lane data).
+// One partner entry v (lane data).
 struct Partner {
     double c, d;
-    uint32_t c180, d180, n180;  // c, d and -d mod 180
-    uint32_t cF, dF, nF;        // their area factors for the pair's class (area_x)
     uint32_t vidx, kj, jj;
     uint32_t cls;           // area class of v (the class rule of hss_math.h)
     bool vc;                // v is the central (real) entry of its half-set
-    LaneCtx ctx;
 };
 
-// The pair context of m = q * k (make_lane_ctx).
-// (make_lane_ctx only asks whether 3 and 5 divide m: the divisibility tests
-// k * 3^-1 <= (2^32 - 1) / 3 and k * 5^-1 <= (2^32 - 1) / 5 modulo 2^32.)
-__device__ inline void pair_ctx(const LeftDesc& L, const PosDesc& S, uint32_t k, LaneCtx& c) {
-    const uint32_t m3 = (L.q3 == 0u || k * 0xAAAAAAABu <= 0x55555555u) ? 0u : 1u;
-    const uint32_t m5 = (L.q5 == 0u || k * 0xCCCCCCCDu <= 0x33333333u) ? 0u : 1u;
-    make_lane_ctx_d((double)L.q * (double)k, S.l4_d, S.l2_d, S.n4_d, S.l3, S.l5, m3, m5, c);
+// The band of the pair (l, m = q k).
+__device__ inline Band pair_band(const LeftDesc& L, const PosDesc& S, uint32_t k) {
+    return make_band((double)L.q * (double)k, S.l4_d, S.l2_d, S.n4_d);
 }
 
-// Per-k record of the widened bounds (the narrowed ones are w -+ tau) and the
-// area-test constants: one per lane of a partner chunk, shared by the k's
+// The class bits the pair (l, q k) constrains (area_constrained) and whether
+// the pair is even (l or m even; q is odd): cm | even << 2.  Whether 3 or 5
+// divides m: the divisibility tests k * 3^-1 <= (2^32 - 1) / 3 and
+// k * 5^-1 <= (2^32 - 1) / 5 modulo 2^32 (or q's own factor).
+__device__ inline uint32_t pair_area_info(const LeftDesc& L, const PosDesc& S, uint32_t k) {
+    const bool m3z = L.q3 == 0u || k * 0xAAAAAAABu <= 0x55555555u;
+    const bool m5z = L.q5 == 0u || k * 0xCCCCCCCDu <= 0x33333333u;
+    const uint32_t cm = area_constrained(S.l3 == 0u || m3z, S.l5 != 0u && !m5z);
+    return cm | ((S.l_even || !(k & 1u)) ? 4u : 0u);
+}
+
+// Per-k record of the widened bounds (the narrowed ones are w - tau) and the
+// pair's area info: one per lane of a partner chunk, shared by the k's
 // half-set entries.
 struct KCtx {
     double mid, half_w, h2_w, tau;
-    uint32_t inv24, thr22;
+    uint32_t info;  // pair_area_info
+    uint32_t pad;
 };
-
-__device__ inline void ctx_from_k(const KCtx& r, LaneCtx& c) {
-    c.mid = r.mid;
-    c.half_w = r.half_w;
-    c.h2_w = r.h2_w;
-    c.tau = r.tau;
-    c.half_n = r.half_w - r.tau;
-    c.h2_n = r.h2_w - r.tau;
-    c.inv24 = r.inv24;
-    c.thr22 = r.thr22;
-    c.lm = 0.0;  // unused by the fast path (the queue recomputes it)
-}
 
 __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, uint32_t jj, uint32_t Hj,
                                     uint32_t oj, Partner& v) {
@@ -679,37 +619,41 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
     v.vc = jj + 1 == Hj;
     v.c = P.cre[v.vidx];
     v.d = P.cim[v.vidx];
-    const uint32_t vr = P.cres[v.vidx];
-    v.c180 = vr & 0xFFu;
-    v.cls = (vr >> CRES_CLASS_SHIFT) & 3u;
-    v.d180 = vr >> 16;
-    v.n180 = v.d180 ? QA - v.d180 : 0u;
+    v.cls = (P.cres[v.vidx] >> CRES_CLASS_SHIFT) & 3u;
 }
 
-// The area factors of v's residues, once v.ctx holds the pair's class.
-__device__ inline void partner_factors(Partner& v) {
-    v.cF = mul_u24(v.c180, v.ctx.inv24);
-    v.dF = mul_u24(v.d180, v.ctx.inv24);
-    v.nF = mul_u24(v.n180, v.ctx.inv24);
-}
-
-// The central (real) entry u of the left set against v: leaf u*v only; with a
-// real v too, the real-leaf rule (its count goes to cnt, not through the queue).
-template <bool ZZ = false>
-__device__ inline void central_leaf(const CellParams& P, RootQueue& rq, const TileCtx& tc, bool act,
-                                    const LeftFast& u, uint32_t ui, const Partner& v, uint32_t& cnt) {
-    const double t1 = u.im * v.d, t2 = u.re * v.d;
-    // area products of X = re c - im d and Y = im c + re d, with -d as its
-    // residue: residue sums below 2 * 180^2 < 2^16, no subtraction
-    const uint32_t xX = area_x(u.ra8, v.cF, u.rb8, v.nF);
-    const uint32_t xY = area_x(u.ra8, v.dF, u.rb8, v.cF);
-    const bool real = act && v.vc;
-    const bool a2 = real && area_ok_x(xX, v.ctx);
-    // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
-    cnt += (a2 && v.ctx.h2_n >= 0.0) ? 1u : 0u;
-    push_global(P, a2 && v.ctx.h2_w >= 0.0 && v.ctx.h2_n < 0.0, DefItem{tc.left, ui, v.vidx, v.kj | 0x80000000u});
-    leaf_fast<ZZ>(P, rq, tc, act & !v.vc, __builtin_fma(u.re, v.c, -t1), __builtin_fma(u.im, v.c, t2), xX, xY,
-                  v.ctx, v.kj, pack_ui(ui, v.jj));
+// The leaf c_u * v of the left set's central (real) entry with partner entry
+// v by the class rule: orientation 1 (d = c_u Re v) counts only when v's class
+// is 0 on the constrained bits cm, orientation 2 (d = c_u Im v) only on even
+// pairs with class 3 on cm.  A real v makes the leaf real: orientation 1 has
+// area 0, orientation 2 has d = 0 and counts once when lo <= 0.  Counted terms
+// are root-tested here; an undecided leaf goes whole to the exact stage.
+template <bool ONE>
+__device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
+                                   const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t& cnt) {
+    const double X = cu * v.c, Y = cu * v.d;
+    const bool ok1 = act && !v.vc && (v.cls & cm) == 0u;
+    const bool ok2 = act && even && ((v.cls ^ 3u) & cm) == 0u;
+    const bool real2 = ok2 && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
+    const double h2n = b.h2_w - b.tau;
+    bool u1, u2;
+    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
+    const uint32_t f2 = o1_decide<ONE>(ok2 && !v.vc, Y, b, u2);
+    const bool und = u1 | u2 | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
+    if (__builtin_amdgcn_ballot_w64(und)) push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | DEF_FULL});
+    const uint32_t f = und ? 0u : (f1 | f2 << 2);
+    cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
+    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
+    bool pass = false;
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+        if (ONE && (t & 1u)) continue;
+        const bool on = (f >> t) & 1u;
+        if (!__builtin_amdgcn_ballot_w64(on)) continue;
+        const double a = t < 2u ? ax : ay;
+        pass |= on && root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, a, b.lm));
+    }
+    if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
 }
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
@@ -801,37 +745,37 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0;
-        // the pair context once per k (a k's half-set entries share it)
+        // the pair's band and area info once per k (a k's half-set entries share them)
         const auto on_chunk = [&](uint32_t k) {
-            LaneCtx c;
-            pair_ctx(L, S, k, c);
+            const Band c = pair_band(L, S, k);
             __builtin_amdgcn_wave_barrier();
-            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, c.inv24, c.thr22};
+            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, pair_area_info(L, S, k), 0u};
             __builtin_amdgcn_wave_barrier();
         };
         for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
                            [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j, uint32_t oj) {
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
-            ctx_from_k(lds_at(kctx, j), v.ctx);
-            partner_factors(v);
+            const KCtx K = lds_at(kctx, j);
+            const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(L.q, kj, S.l4_d)};
+            const uint32_t cm = K.info & 3u;
+            const bool even = (K.info & 4u) != 0u;
             const uint32_t jpk = jj << ID_U_BITS;
             // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
-            // (instantiated for tiles with and without second-sign terms)
+            // (same class: conjugation keeps it), instantiated for tiles with
+            // and without second-sign terms
             const auto rows = [&](auto one_c) {
                 constexpr bool ONE = decltype(one_c)::value;
                 for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
                     const LeftFast u = lf[ui];
+                    const uint32_t c = (u.cls ^ v.cls) & cm;
+                    const bool ok1 = act && c == 0u, ok2 = act && even && c == cm;
                     const double t1 = u.im * v.d, t2 = u.re * v.d;
-                    // area products (central_leaf): u*v has d, u*conj(v) has -d
-                    const uint32_t pac = mul_u24(u.ra8, v.cF), pbc = mul_u24(u.rb8, v.cF);
-                    const uint32_t vX0 = pac + mul_u24(u.rb8, v.nF), vY0 = pbc + mul_u24(u.ra8, v.dF);
-                    const uint32_t vX1 = pac + mul_u24(u.rb8, v.dF), vY1 = pbc + mul_u24(u.ra8, v.nF);
                     const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
                     const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
                     bool u0, u1;
-                    const uint32_t f0 = leaf_decide<ONE>(act, X0, Y0, vX0, vY0, v.ctx, u0);
-                    const uint32_t f1 = leaf_decide<ONE>(act & !v.vc, X1, Y1, vX1, vY1, v.ctx, u1);
+                    const uint32_t f0 = leaf_decide<ONE>(ok1, ok2, X0, Y0, b, u0);
+                    const uint32_t f1 = leaf_decide<ONE>(ok1 & !v.vc, ok2 & !v.vc, X1, Y1, b, u1);
                     if (__builtin_amdgcn_ballot_w64(u0 | u1)) {  // rare
                         defer_leaf(P, tc, u0, kj, ui | jpk);
                         defer_leaf(P, tc, u1, kj | 1u << 27, ui | jpk);
@@ -840,13 +784,16 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                     leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
                 }
             };
-            if (one)
+            const double cu = lf[L.H - 1].re;  // the central entry is real
+            if (one) {
                 rows(std::true_type{});
-            else
+                central_win<true>(P, tc, act, cu, L.H - 1, v, b, cm, even, cnt);
+            } else {
                 rows(std::false_type{});
-            central_leaf(P, rq, tc, act, lf[L.H - 1], L.H - 1, v, cnt);
+                central_win<false>(P, tc, act, cu, L.H - 1, v, b, cm, even, cnt);
+            }
         });
-        root_flush<false>(P, rq, tc);
+        root_flush(P, rq, tc);
         tile_done(P, S, cnt, rq.tested, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
         rq.tested = 0;
     }
@@ -896,40 +843,6 @@ struct alignas(16) WinVp {
                      // the partner lane | rho << 6
 };
 static_assert(sizeof(WinVp) == 48, "LDS budget of cell_win_kernel");
-
-// The leaf c_u * v of the left set's central (real) entry with partner entry
-// v by the class rule: orientation 1 (d = c_u Re v) counts only when v's class
-// is 0 on the constrained bits cm, orientation 2 (d = c_u Im v) only on even
-// pairs with class 3 on cm.  A real v makes the leaf real: orientation 1 has
-// area 0, orientation 2 has d = 0 and counts once when lo <= 0.  Counted terms
-// are root-tested here; an undecided leaf goes whole to the exact stage.
-template <bool ONE>
-__device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
-                                   const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t& cnt) {
-    const double X = cu * v.c, Y = cu * v.d;
-    const bool ok1 = act && !v.vc && (v.cls & cm) == 0u;
-    const bool ok2 = act && even && ((v.cls ^ 3u) & cm) == 0u;
-    const bool real2 = ok2 && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
-    const double h2n = b.h2_w - b.tau;
-    bool u1, u2;
-    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
-    const uint32_t f2 = o1_decide<ONE>(ok2 && !v.vc, Y, b, u2);
-    const bool und = u1 | u2 | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
-    if (__builtin_amdgcn_ballot_w64(und)) push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | DEF_FULL});
-    const uint32_t f = und ? 0u : (f1 | f2 << 2);
-    cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
-    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
-    bool pass = false;
-#pragma unroll
-    for (uint32_t t = 0; t < 4; ++t) {
-        if (ONE && (t & 1u)) continue;
-        const bool on = (f >> t) & 1u;
-        if (!__builtin_amdgcn_ballot_w64(on)) continue;
-        const double a = t < 2u ? ax : ay;
-        pass |= on && root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, a, b.lm));
-    }
-    if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
-}
 
 // Phase 3: the flattened units of a batch of virtual partners.
 template <bool ONE>
@@ -1048,13 +961,10 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             // ---- phase 1: lane = partner entry
             Partner v;
             load_partner(P, act, kj, jj, Hj, oj, v);
-            pair_ctx(L, S, kj, v.ctx);
-            const Band b{v.ctx.mid, v.ctx.half_w, v.ctx.h2_w, v.ctx.tau, v.ctx.lm};
-            // the pair's constrained class bits (3 | m and 5 | m by inverse multiplication)
-            const bool m3z = L.q3 == 0u || kj * 0xAAAAAAABu <= 0x55555555u;
-            const bool m5z = L.q5 == 0u || kj * 0xCCCCCCCDu <= 0x33333333u;
-            const uint32_t cm = area_constrained(S.l3 == 0u || m3z, S.l5 != 0u && !m5z);
-            const bool even = S.l_even || !(kj & 1u);  // m = q k with q odd
+            const Band b = pair_band(L, S, kj);
+            const uint32_t ainfo = pair_area_info(L, S, kj);
+            const uint32_t cm = ainfo & 3u;
+            const bool even = (ainfo & 4u) != 0u;
             if (one)
                 central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, cnt);
             else

@@ -15,9 +15,10 @@
 //
 // Fast path.  Every z is 2*u*v (or 2*u*conj(v)) with u from a "left set" of
 // the root l and v from the half-set of a partner k; u and v carry FP64
-// coordinates and exact residues modulo 180 = 4 * 9 * 5.  Writing z/2 = X + iY:
+// coordinates and a two-bit area class.  Writing z/2 = X + iY:
 //   * the area tests need X or Y modulo 4 and modulo h = (3 or 9)*(1 or 5),
-//     decided exactly from the residues;
+//     decided exactly from the classes of u and v and the parity of l m (the
+//     class rule below);
 //   * the range test is decided from FP64 estimates whose error is provably
 //     below tau/16; an estimate within tau of a bound is "undecided" and the
 //     whole leaf goes to the exact stage;
@@ -103,31 +104,23 @@ HSS_HD inline bool bit_of(const uint32_t* words, uint32_t i) { return (words[i >
 
 // ------------------------------------------------- per-(l, m) fast-path state
 
-struct LaneCtx {
-    // range bounds on |D| (halved bounds on |2D|: halving is exact), widened (w)
-    // and narrowed (n) by tau / 2:
-    //   term1: lo+ <= |2D| <= hi (d = +|2D|),  term2: |2D| <= -lo (d = -|2D|)
-    double h1_lo_w, h1_lo_n, h1_hi_w, h1_hi_n;
-    double h2_w, h2_n;
-    // term1 as one distance: |D| in [h1_lo, h1_hi] <=> ||D| - mid| <= half with
-    // mid = (lo+ + hi) / 4, half = (hi - lo+) / 4 widened (half_w) and
-    // narrowed (half_n) by tau / 2; mid, half and the distance add at most five
-    // roundings of 2^-52 N^4 <= tau / 256 each to the tau / 16 budget, still
-    // far inside the tau / 2 band
-    double mid, half_w, half_n;
-    double tau;   // undecided band: FP64 errors stay below tau / 16
-    double lm;    // FP64 value of l^4 + m^4
-    uint32_t inv24;  // inverse of h = (3 or 9) * (1 or 5) modulo 2^24
-    uint32_t thr22;  // floor((2^22 - 1) / h)
+// The range test of a pair (l, m) as a band on |D| for D = X or Y of a leaf
+// (the halved bounds on |2D|: halving is exact), with lo = l^4 + m^4 - N^4 and
+// hi = l^4:
+//   term 1 (d = +|2D|): lo+ <= |2D| <= hi, i.e. ||D| - mid| <= half with
+//     mid = (lo+ + hi) / 4, half = (hi - lo+) / 4, lo+ = max(lo, 0);
+//   term 2 (d = -|2D|): |2D| <= -lo.
+// Both are widened by tau / 2 (half_w, h2_w); the narrowed bounds are w - tau.
+// An estimate between the two is undecided.  mid, half and the distance add at
+// most five roundings of 2^-52 N^4 <= tau / 256 each to the tau / 16 budget,
+// still far inside the tau / 2 band.
+struct Band {
+    double mid, half_w, h2_w;
+    double tau;  // undecided band: FP64 errors stay below tau / 16
+    double lm;   // FP64 value of l^4 + m^4 (the zz estimates are lm -+ 2|D|)
 };
 
-HSS_HD constexpr uint32_t inv_mod_2_24(uint32_t h) {
-    uint32_t x = h;  // Newton iteration for the inverse of an odd h modulo 2^24
-    for (int i = 0; i < 5; ++i) x *= 2u - h * x;
-    return x & 0xFFFFFFu;
-}
-
-// FP64 value of l^4 + m^4 (the zz estimates of a pair are lm -+ 2|D|)
+// FP64 value of l^4 + m^4
 HSS_HD inline double lm_of(uint64_t m, double l4_d) {
     const double md = (double)(uint32_t)m;  // exact: m < 2^26
     const double m2 = md * md;
@@ -141,79 +134,23 @@ HSS_HD inline double lm_of(uint32_t q, uint32_t k, double l4_d) {
     return l4_d + m2 * m2;
 }
 
-// Per-pair context.  l4_d, n4_d: FP64 values of l^4 and N^4; l2_d = l^2;
-// l3 = l mod 3, l5 = l mod 5; m3 = m mod 3, m5 = m mod 5 (only whether they
-// are 0 matters); md = m as an FP64 value (exact: m < 2^26).
-HSS_HD inline void make_lane_ctx_d(double md, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
-                                   uint32_t m3, uint32_t m5, LaneCtx& c) {
-    double m2 = md * md;                     // exact: m^2 < 2^52
-    double m4 = m2 * m2;
-    double lo = (l4_d + m4) - n4_d;
-    double hi = l4_d;
-    double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
-    double lop = lo > 0.0 ? lo : 0.0;
-    c.h1_lo_w = (lop - tau) * 0.5;
-    c.h1_lo_n = (lop + tau) * 0.5;
-    c.h1_hi_w = (hi + tau) * 0.5;
-    c.h1_hi_n = (hi - tau) * 0.5;
-    c.mid = (lop + hi) * 0.25;
-    c.half_w = (hi - lop) * 0.25 + tau * 0.5;
-    c.half_n = c.half_w - tau;
-    c.h2_w = (-lo + tau) * 0.5;
-    c.h2_n = (-lo - tau) * 0.5;
-    c.tau = tau;
-    c.lm = l4_d + m4;
-    // h = (9 if 3 | l or 3 | m, else 3) * (5 if 5 divides neither, else 1)
-    const bool need9 = l3 == 0 || m3 == 0;
-    const bool need5 = l5 != 0 && m5 != 0;
-    constexpr uint32_t I3 = inv_mod_2_24(3), I9 = inv_mod_2_24(9), I15 = inv_mod_2_24(15), I45 = inv_mod_2_24(45);
-    c.inv24 = need9 ? (need5 ? I45 : I9) : (need5 ? I15 : I3);
-    c.thr22 = need9 ? (need5 ? 0x3FFFFFu / 45 : 0x3FFFFFu / 9) : (need5 ? 0x3FFFFFu / 15 : 0x3FFFFFu / 3);
+// The band of the pair (l, m): l4_d, n4_d the FP64 values of l^4 and N^4,
+// l2_d = l^2, md = m as an FP64 value (exact: m < 2^26).
+HSS_HD inline Band make_band(double md, double l4_d, double l2_d, double n4_d) {
+    const double m2 = md * md;  // exact: m^2 < 2^52
+    const double m4 = m2 * m2;
+    const double lo = (l4_d + m4) - n4_d;
+    const double hi = l4_d;
+    const double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
+    const double lop = lo > 0.0 ? lo : 0.0;
+    Band b;
+    b.mid = (lop + hi) * 0.25;
+    b.half_w = (hi - lop) * 0.25 + tau * 0.5;
+    b.h2_w = (-lo + tau) * 0.5;
+    b.tau = tau;
+    b.lm = l4_d + m4;
+    return b;
 }
-
-HSS_HD inline void make_lane_ctx(uint64_t m, double l4_d, double l2_d, double n4_d, uint32_t l3, uint32_t l5,
-                                 uint32_t m3, uint32_t m5, LaneCtx& c) {
-    make_lane_ctx_d((double)(uint32_t)m, l4_d, l2_d, n4_d, l3, l5, m3, m5, c);
-}
-
-// area4 = 2A: area4 = 0 (mod 8) and |area4| = 0 (mod h), from any va = A (mod
-// 180) with 0 <= va < 2^16, given as vs = va << 8.  With x = va * h^-1 mod 2^24,
-// va = 0 (mod 4h) iff the 24-bit rotation of x right by 2 is at most
-// (2^22 - 1) / h (divisibility by an odd h through its inverse, the factor 4
-// through the rotated-out low bits).  vs * inv24 mod 2^32 = x << 8, so the
-// rotation is a 32-bit rotation by 10 of that product.
-HSS_HD inline bool area_ok_x(uint32_t x, const LaneCtx& c) { return ((x >> 10) | (x << 22)) <= c.thr22; }
-
-HSS_HD inline bool area_ok_s(uint32_t vs, const LaneCtx& c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t x;  // both operands are below 2^24: one 24-bit multiply, no masking
-    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(x) : "v"(vs), "v"(c.inv24));
-#else
-    const uint32_t x = vs * c.inv24;
-#endif
-    return area_ok_x(x, c);
-}
-
-// The low 32 bits of the product of the low 24 bits of a and b (v_mul_u32_u24).
-HSS_HD inline uint32_t mul_u24(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umul24(a, b);
-#else
-    return (uint32_t)((uint64_t)(a & 0xFFFFFFu) * (uint64_t)(b & 0xFFFFFFu));
-#endif
-}
-
-// The area test's x without its multiply by h^-1: with residues r (mod 180)
-// stored as r8 = r << 8 and a partner residue r' stored as its area factor
-// F = r' * inv24 (any representative of r' h^-1 mod 2^24),
-// mul_u24(r8, F) = (r r' h^-1 mod 2^24) << 8 (mod 2^32), and the sum of two
-// such products is x = va * h^-1 << 8 (mod 2^32) for va = r1 r1' + r2 r2'
-// (< 2^16), exactly what area_ok_s forms from va << 8.
-HSS_HD inline uint32_t area_x(uint32_t r8a, uint32_t Fa, uint32_t r8b, uint32_t Fb) {
-    return mul_u24(r8a, Fa) + mul_u24(r8b, Fb);
-}
-
-HSS_HD inline bool area_ok(uint32_t va, const LaneCtx& c) { return area_ok_s(va << 8, c); }
 
 // Do all pairs (l, q k), k >= k0, have lo > tau?  Then h2_w = (tau - lo)/2 < 0
 // and the second-sign terms (term2 of both orientations: d = -|2D|, i.e. zz =
@@ -227,19 +164,6 @@ HSS_HD inline bool tile_one_term(uint32_t q, uint32_t k0, double l4_d, double l2
     const double lo = (l4_d + m2 * m2) - n4_d;
     const double tau = (4.0 * (l2_d * m2) + n4_d) * 0x1p-44;
     return lo > 2.0 * tau;
-}
-
-// Range terms of one orientation with d = 2D: term1 counts d = +|2D|, term2
-// d = -|2D|.  undecided is set when the FP64 estimate is within tau of a bound.
-HSS_HD inline void range_terms(double Dd, const LaneCtx& c, bool& t1, bool& t2, bool& undecided) {
-    double x = Dd < 0 ? -Dd : Dd;
-    bool t1w = x >= c.h1_lo_w && x <= c.h1_hi_w;
-    bool t1n = x >= c.h1_lo_n && x <= c.h1_hi_n;
-    bool t2w = x <= c.h2_w;
-    bool t2n = x <= c.h2_n;
-    undecided = (t1w && !t1n) || (t2w && !t2n);  // the narrowed ranges lie inside the widened ones
-    t1 = t1n;
-    t2 = t2n;
 }
 
 // can an FP64 estimate zz (relative error < 2^-49, zz >= 1) be an exact fourth power?
@@ -282,62 +206,12 @@ HSS_HD inline bool root_near_integer(double zz) {
 #endif
 }
 
-// One leaf z/2 = X + iY with area residues vX = X, vY = Y (mod 180).  cnt is
-// the decided candidate count; und is set when the leaf must be counted by the
-// exact stage instead; f[t] marks counted terms (orientation 1: term1, term2;
-// orientation 2: term1, term2) and zz[t] their FP64 zz estimates.  The two
-// orientations coincide in magnitude (eq) when min(|X - Y|, |X + Y|) =
-// ||X| - |Y|| (the same FP64 value either way) is within tau.
-struct LeafEval {
-    uint32_t cnt;
-    bool und;
-    bool f[4];
-    double zz[4];
-};
-
-HSS_HD inline void eval_leaf(double X, double Y, uint32_t vX, uint32_t vY, const LaneCtx& c, LeafEval& r) {
-    bool a1 = area_ok(vY, c);
-    bool a2 = area_ok(vX, c);
-    double ax = X < 0 ? -X : X, ay = Y < 0 ? -Y : Y;
-    double dd = ax - ay;
-    bool eq = (dd < 0 ? -dd : dd) <= c.tau;
-    bool t1, t2, u1, t3, t4, u2;
-    range_terms(X, c, t1, t2, u1);
-    range_terms(Y, c, t3, t4, u2);
-    bool und = (a1 && (u1 || ay <= c.tau)) || (a2 && (u2 || eq || ax <= c.tau));
-    bool o2 = a2 && !eq;
-    r.f[0] = a1 && t1;
-    r.f[1] = a1 && t2;
-    r.f[2] = o2 && t3;
-    r.f[3] = o2 && t4;
-    r.cnt = (uint32_t)r.f[0] + (uint32_t)r.f[1] + (uint32_t)r.f[2] + (uint32_t)r.f[3];
-    r.und = und;
-    r.zz[0] = c.lm - 2.0 * ax;
-    r.zz[1] = c.lm + 2.0 * ax;
-    r.zz[2] = c.lm - 2.0 * ay;
-    r.zz[3] = c.lm + 2.0 * ay;
-}
-
-// The leaf u*v with both factors real (the central entries): z = 2uv is real,
-// so orientation 1 has area4 = 0 (never counted) and orientation 2 has d = 0,
-// counted once when area4 = 2uv passes the area tests and lo <= 0 (only the
-// positive sign: d = 0).  zz = l^4 + m^4 is never a fourth power (Fermat, n = 4),
-// so no term needs the root test.  vX = uv (mod 180).
-HSS_HD inline void eval_real_leaf(uint32_t vX, const LaneCtx& c, LeafEval& r) {
-    bool a2 = area_ok(vX, c);
-    // -lo - tau >= 0  <=>  lo <= -tau (decided);  |lo| <= tau: undecided
-    r.cnt = (a2 && c.h2_n >= 0.0) ? 1u : 0u;
-    r.und = a2 && c.h2_w >= 0.0 && c.h2_n < 0.0;
-    r.f[0] = r.f[1] = r.f[2] = r.f[3] = false;
-    r.zz[0] = r.zz[1] = r.zz[2] = r.zz[3] = 0.0;
-}
-
 // ----------------------------------------------------------- range windows
 //
 // A leaf z = 2uv has |z| = R = 2 l^2 m^2, so its range terms depend only on
 // psi = arg z modulo pi/2 (the four terms are invariant under z -> iz and
 // z -> conj z).  With S = [s0, s1] the set of |D| / R that can pass or be
-// undecided (range_terms widened by tau), a term can count only when
+// undecided (the Band widened by tau), a term can count only when
 // sin psi or cos psi lies in S:
 //     psi in A = [asin s0, asin s1]  or  psi in A' = pi/2 - A.
 // Left sets in window mode are sorted by psi_u; since psi(uv) = psi_u + psi_v
@@ -408,7 +282,7 @@ HSS_HD inline uint32_t psi_bucket(double t) {
 }
 
 // psi window of the pair (l, m): n = 1 or 2 intervals [a_i, b_i], or n = -1
-// for the whole circle.  lo, hi, tau as in make_lane_ctx; R = 2 l^2 m^2.
+// for the whole circle.  lo, hi, tau as in make_band; R = 2 l^2 m^2.
 struct Window {
     double a0, b0, a1, b1;
     int n;
@@ -597,7 +471,7 @@ HSS_HD inline uint32_t window_ranges(const Window& w, const uint32_t* bt, uint32
 // divisible by 3, Im iff c3 = 0 (by 5 when 5 does not divide l m, Im iff
 // c5 = 0).  The area tests of the reference (src/search_v18.cpp:441-453) then
 // reduce to classes and parities, for pairs whose area modulus h = (3 or 9) *
-// (1 or 5) (make_lane_ctx):
+// (1 or 5):
 //   orientation 1 (d = Re z, area4 = Im z, 2^3 | 2 F Im z' always):
 //     counts only if (c3 = 0 or 3 | l m) and (c5 = 0 or 5 | l m);
 //   orientation 2 (d = Im z, area4 = Re z = 2 F Re z', Re z' odd): counts only
@@ -752,20 +626,14 @@ HSS_HD inline uint32_t window_o1_ranges(const Window1& w, const uint32_t* btg, u
     return (e[0] - s[0]) + (e[1] - s[1]) + (e[2] - s[2]) + (e[3] - s[3]);
 }
 
-// Band of one window record: the pair's term-1 distance band (widened; the
-// narrowed bounds are w - tau), term-2 bound, l^4 + m^4.
-struct Band {
-    double mid, half_w, h2_w, tau, lm;
-};
-
 // Orientation-1 terms of a leaf with X = Re(u w) (the class rule already
 // holds): bit 0 term 1 (d = +|2X|), bit 1 term 2 (d = -|2X|, not on ONE tiles);
 // und when the FP64 band leaves the leaf undecided (then no term is returned).
 template <bool ONE>
 HSS_HD inline uint32_t o1_decide(bool live, double X, const Band& b, bool& und) {
-    const double ax = X < 0 ? -X : X;
-    const double dx = ax - b.mid;
-    const double adx = dx < 0 ? -dx : dx;
+    // fabs (not a select): an absolute-value source modifier on the device
+    const double ax = __builtin_fabs(X);
+    const double adx = __builtin_fabs(ax - b.mid);
     const bool t1w = adx <= b.half_w, t1n = adx <= b.half_w - b.tau;
     const bool t2w = !ONE && ax <= b.h2_w, t2n = !ONE && ax <= b.h2_w - b.tau;
     const bool u = live & ((t1w != t1n) | (t2w != t2n));

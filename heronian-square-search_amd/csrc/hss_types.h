@@ -88,7 +88,8 @@ struct TileDesc {
 // FP64 view of a left entry used by the hot loop (one scalar load).
 struct LeftFast {
     double re, im;  // FP64 values (exact below 2^53, else correctly rounded)
-    uint32_t ra8, rb8;  // (re mod 180) << 8, (im mod 180) << 8
+    uint32_t cls;   // area class (the class rule of hss_math.h)
+    uint32_t pad;
 };
 
 // Sizes of one batch, read by the kernels from device memory so that the

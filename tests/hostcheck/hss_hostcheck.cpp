@@ -160,7 +160,8 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
     // left sets; sorted ones (window mode) ordered by psi with the central entry last
     std::vector<i128> lre(P.left_entries), lim(P.left_entries);
     std::vector<LeftFast> lfast(P.left_entries);
-    std::vector<uint32_t> lcls(P.left_entries);  // class of each stored left entry (checks only)
+    std::vector<uint32_t> lcls(P.left_entries);  // class of each stored left entry
+    std::vector<uint32_t> lres(P.left_entries);  // (re mod 180) | (im mod 180) << 16 (class-rule check only)
     std::vector<uint32_t> btab(P.sorted.size() * (WIN_NB + 1));
     for (size_t li = 0; li < P.lefts.size(); ++li) {
         const LeftBuild& b = P.builds[li];
@@ -204,10 +205,11 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             LeftFast f;
             f.re = i128_to_double(z.re);
             f.im = i128_to_double(z.im);
-            f.ra8 = smod128(z.re, QA) << 8;
-            f.rb8 = smod128(z.im, QA) << 8;
+            f.cls = cls[perm[t]];
+            f.pad = 0;
             lfast[L.off + t] = f;
             lcls[L.off + t] = cls[perm[t]];
+            lres[L.off + t] = smod128(z.re, QA) | smod128(z.im, QA) << 16;
         }
     }
     std::fill(cand, cand + n_roots, 0);
@@ -218,56 +220,39 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         uint32_t left, ui, vidx, k, conj, full, omask;
     };
     std::vector<Def> defs;
-    // one leaf: u (entry ui of left set li) times v or conj(v) (cj)
-    auto leaf = [&](uint32_t li, const LeftDesc& L, const PosDesc& D, const LaneCtx& ctx, uint32_t ui, uint32_t vidx,
-                    uint32_t k, uint32_t cj, bool both_real) {
-        const LeftFast& u = lfast[L.off + ui];
-        const double c = T.cre[vidx], d = T.cim[vidx];
+    // The class rule against the reference's area test on residues modulo 180
+    // (src/search_v18.cpp:441-453: 4h | area4 / 2, h = (9 if 3 | l m, else 3) *
+    // (5 if 5 divides neither, else 1)) for the leaf u * v or u * conj(v) (cj):
+    // orientation 1 tests Im z, orientation 2 Re z.  A real leaf (both factors
+    // central) has orientation-1 area 0 and is not compared there.
+    auto check_class = [&](const LeftDesc& L, const PosDesc& D, uint32_t ui, uint32_t vidx, uint32_t k, uint32_t cj,
+                           bool both_real) {
+        const uint32_t ua = lres[L.off + ui] & 0xFFFFu, ub = lres[L.off + ui] >> 16;
         const uint32_t c180 = T.cres[vidx] & 0xFFu, d180 = T.cres[vidx] >> 16;
-        const uint32_t ua = u.ra8 >> 8, ub = u.rb8 >> 8;
-        const double t1 = u.im * d, t2 = u.re * d;
         const uint32_t pac = ua * c180, pbd = ub * d180, pad = ua * d180, pbc = ub * c180;
-        ++S.cells;
-        double X, Y;
-        uint32_t vX, vY;
-        if (cj == 0) {
-            X = __builtin_fma(u.re, c, -t1);
-            Y = __builtin_fma(u.im, c, t2);
-            vX = pac + QA * QA - pbd;
-            vY = pad + pbc;
-        } else {
-            X = __builtin_fma(u.re, c, t1);
-            Y = __builtin_fma(u.im, c, -t2);
-            vX = pac + pbd;
-            vY = pbc + QA * QA - pad;
+        const uint32_t vX = cj ? pac + pbd : pac + QA * QA - pbd;
+        const uint32_t vY = cj ? pbc + QA * QA - pad : pad + pbc;
+        const uint64_t m = (uint64_t)L.q * k;
+        const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
+        const uint32_t h4 = 4u * (need9 ? 9u : 3u) * (need5 ? 5u : 1u);
+        const uint32_t cm = area_constrained(need9, need5);
+        const uint32_t c = (lcls[L.off + ui] ^ ((T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u)) & cm;
+        const bool even = D.l_even || !(k & 1u);
+        const bool rule1 = c == 0u, rule2 = even && c == cm;
+        if ((!both_real && rule1 != (vY % h4 == 0)) || rule2 != (vX % h4 == 0)) ++S.class_mismatch;
+    };
+    // a decided leaf's counted terms f (bits: orientation 1 term 1, 2 on |X|,
+    // orientation 2 on |Y|) go to the root test; one survivor defers the leaf
+    auto root_test = [&](uint32_t li, uint32_t ui, uint32_t vidx, uint32_t k, uint32_t cj, uint32_t f, double X,
+                         double Y, double lm) {
+        const double ax = std::fabs(X), ay = std::fabs(Y);
+        bool pass = false;
+        for (uint32_t t = 0; t < 4; ++t)
+            pass |= ((f >> t) & 1u) && root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, t < 2u ? ax : ay, lm));
+        if (pass) {
+            defs.push_back({li, ui, vidx, k, cj, 0, 3});
+            ++S.deferred_root;
         }
-        LeafEval r;
-        if (both_real)
-            eval_real_leaf(vX, ctx, r);
-        else
-            eval_leaf(X, Y, vX, vY, ctx, r);
-        // the class rule against the residue test of both orientations (area 0 aside)
-        {
-            const uint64_t mm = (uint64_t)L.q * k;
-            const uint32_t cm = area_constrained(D.l3 == 0 || mm % 3 == 0, D.l5 != 0 && mm % 5 != 0);
-            const uint32_t c = (lcls[L.off + ui] ^ ((T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u)) & cm;
-            const bool even = D.l_even || !(k & 1u);
-            const bool rule1 = c == 0u, rule2 = even && c == cm;
-            const bool res1 = area_ok(vY, ctx), res2 = area_ok(vX, ctx);
-            // orientation 1 of a real leaf has area 0 (the residue test sees 0 as divisible)
-            if ((!both_real && rule1 != res1) || rule2 != res2) ++S.class_mismatch;
-        }
-        if (r.und) {
-            defs.push_back({li, ui, vidx, k, cj, 1, 3});
-            ++S.deferred_full;
-            return;
-        }
-        cand[D.out] += r.cnt;
-        for (int tm = 0; tm < 4; ++tm)
-            if (r.f[tm] && root_near_integer(r.zz[tm])) {
-                defs.push_back({li, ui, vidx, k, cj, 0, 3});
-                ++S.deferred_root;
-            }
     };
     for (int mode = 0; mode < 2; ++mode) {
         const std::vector<TileDesc>& tiles = mode == 0 ? P.tiles : P.wtiles;
@@ -275,6 +260,10 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
             const LeftDesc& L = P.lefts[Tl.left];
             const PosDesc& D = P.pos[L.pos];
             S.window_tiles += mode;
+            const bool one = tile_one_term(L.q, Tl.k0, D.l4_d, D.l2_d, D.n4_d);
+            const uint32_t kmask = left_kmask(L);
+            const uint32_t nbg = WIN_NB >> __builtin_popcount(kmask);
+            const uint32_t* bt = L.bt != NO_TABLE ? btab.data() + (size_t)L.bt * (WIN_NB + 1) : nullptr;
             for (uint32_t k = Tl.k0; k <= Tl.k1; ++k) {
                 bool ok = !(D.l_even && !(k & 1u));
                 for (uint32_t j = 0; j < D.nsplit; ++j) ok = ok && !(k * D.split_inv[j] <= D.split_thr[j]);
@@ -283,37 +272,22 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                 uint32_t Hk = T.off[k + 1] - T.off[k];
                 if ((L.flags & LEFT_TRIVIAL) && Hk == 1) continue;
                 pairs[D.out] += 1;
-                uint64_t m = (uint64_t)L.q * k;
-                LaneCtx ctx;
-                make_lane_ctx(m, D.l4_d, D.l2_d, D.n4_d, D.l3, D.l5, (uint32_t)(m % 3), (uint32_t)(m % 5), ctx);
+                const uint64_t m = (uint64_t)L.q * k;
+                const double m2 = (double)(m * m);
+                const Band bd = make_band((double)m, D.l4_d, D.l2_d, D.n4_d);
+                const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
+                const uint32_t cm = area_constrained(need9, need5);
+                const bool even = D.l_even || !(k & 1u);
                 for (uint32_t jj = 0; jj < Hk; ++jj) {
                     const uint32_t vidx = T.off[k] + jj;
                     const bool vc = jj + 1 == Hk;
-                    leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
-                    // the central u: u*v only (real-leaf rule when v is real too)
-                    if (mode == 0) leaf(Tl.left, L, D, ctx, L.H - 1, vidx, k, 0, vc);
-                    if (mode == 0) {
-                        for (uint32_t ui = 0; ui + 1 < L.H; ++ui)
-                            for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) leaf(Tl.left, L, D, ctx, ui, vidx, k, cj, false);
-                        continue;
-                    }
-                    // cell_win_kernel: passes over (class variant j, orientation rho)
-                    const double m2 = (double)(m * m);
-                    Window1 w;
-                    make_window_o1((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, ctx.tau, 2.0 * D.l2_d * m2, w);
                     const double c = T.cre[vidx], d = T.cim[vidx];
-                    const double thv = theta_of_f(c, d);
                     const uint32_t vcls = (T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u;
-                    const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
-                    const uint32_t cm = area_constrained(need9, need5), kmask = left_kmask(L);
-                    const uint32_t nbg = WIN_NB >> __builtin_popcount(kmask);
-                    const bool even = D.l_even || !(k & 1u);
-                    const bool one = tile_one_term(L.q, Tl.k0, D.l4_d, D.l2_d, D.n4_d);
-                    const Band bd{ctx.mid, ctx.half_w, ctx.h2_w, ctx.tau, ctx.lm};
-                    const uint32_t* bt = btab.data() + (size_t)L.bt * (WIN_NB + 1);
+                    leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
                     {
-                        // central_win: the central u by the class rule
+                        // central_win: the central u by the class rule (both kernels)
                         ++S.cells;
+                        check_class(L, D, L.H - 1, vidx, k, 0, vc);
                         const double cu = lfast[L.off + L.H - 1].re;
                         const double X = cu * c, Y = cu * d;
                         const bool ok1 = !vc && (vcls & cm) == 0u, ok2 = even && ((vcls ^ 3u) & cm) == 0u;
@@ -330,17 +304,38 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                         } else {
                             const uint32_t f = f1 | f2 << 2;
                             cand[D.out] += __builtin_popcount(f) + ((real2 && h2n >= 0.0) ? 1u : 0u);
-                            const double ax = std::fabs(X), ay = std::fabs(Y);
-                            bool pass = false;
-                            for (uint32_t t = 0; t < 4; ++t)
-                                pass |= ((f >> t) & 1u) &&
-                                        root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, t < 2u ? ax : ay, bd.lm));
-                            if (pass) {
-                                defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 0, 3});
-                                ++S.deferred_root;
-                            }
+                            root_test(Tl.left, L.H - 1, vidx, k, 0, f, X, Y, bd.lm);
                         }
                     }
+                    if (mode == 0) {
+                        // cell_kernel rows: every non-central u, leaves u*v and (v not real) u*conj(v)
+                        for (uint32_t ui = 0; ui + 1 < L.H; ++ui)
+                            for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
+                                ++S.cells;
+                                check_class(L, D, ui, vidx, k, cj, false);
+                                const LeftFast& u = lfast[L.off + ui];
+                                const uint32_t cr = (u.cls ^ vcls) & cm;
+                                const bool ok1 = cr == 0u, ok2 = even && cr == cm;
+                                const double t1 = u.im * d, t2 = u.re * d;
+                                const double X = cj ? __builtin_fma(u.re, c, t1) : __builtin_fma(u.re, c, -t1);
+                                const double Y = cj ? __builtin_fma(u.im, c, -t2) : __builtin_fma(u.im, c, t2);
+                                bool u1, u2;
+                                const uint32_t f1 = one ? o1_decide<true>(ok1, X, bd, u1) : o1_decide<false>(ok1, X, bd, u1);
+                                const uint32_t f2 = one ? o1_decide<true>(ok2, Y, bd, u2) : o1_decide<false>(ok2, Y, bd, u2);
+                                if (u1 || u2) {
+                                    defs.push_back({Tl.left, ui, vidx, k, cj, 1, 3});
+                                    ++S.deferred_full;
+                                    continue;
+                                }
+                                cand[D.out] += __builtin_popcount(f1 | f2 << 2);
+                                root_test(Tl.left, ui, vidx, k, cj, f1 | f2 << 2, X, Y, bd.lm);
+                            }
+                        continue;
+                    }
+                    // cell_win_kernel: passes over (class variant j, orientation rho)
+                    Window1 w;
+                    make_window_o1((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, bd.tau, 2.0 * D.l2_d * m2, w);
+                    const double thv = theta_of_f(c, d);
                     for (uint32_t j = 0; j < allowed_count(cm, kmask); ++j)
                         for (uint32_t rho = 0; rho < (even ? 2u : 1u); ++rho) {
                             const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), cm, kmask, j), kmask);
@@ -371,12 +366,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                                             continue;
                                         }
                                         cand[D.out] += __builtin_popcount(f);
-                                        const double ax = std::fabs(X);
-                                        if (((f & 1u) && root_near_integer(__builtin_fma(-2.0, ax, bd.lm))) ||
-                                            ((f & 2u) && root_near_integer(__builtin_fma(2.0, ax, bd.lm)))) {
-                                            defs.push_back({Tl.left, ui, vidx, k, cj, 0, 3});
-                                            ++S.deferred_root;
-                                        }
+                                        root_test(Tl.left, ui, vidx, k, cj, f, X, 0.0, bd.lm);
                                     }
                             }
                         }
@@ -419,56 +409,54 @@ int hc_search(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_r
     return hc_search_mixed(nmax, roots, limits.data(), n_roots, cand, leaves, pairs, hits, hit_cap, n_hits, st);
 }
 
-// The fast path's area test (one 24-bit multiply and a rotation) against
-// divisibility by 4h, for every residue value the kernels form (below 2^16)
-// and every (l mod 3, l mod 5, m mod 3, m mod 5); returns the mismatches.
-uint64_t hc_area_mismatches(void) {
+// The class rule (hss_math.h, value_class in hss_build.h) on `samples` random
+// products z of values pi^r conj(pi)^(4E - r) of split primes p != 5 (2 or 3
+// factors, E <= 2; |z| < 2^88): Re z odd, 4 | Im z, and exactly one of Re z, Im z divisible
+// by 3 -- Im iff class bit 0 is clear -- and by 5 -- Im iff class bit 1 is
+// clear.  Also z times the conjugate of a product w of two values (E = 1)
+// has the xor of their classes (|z conj w| < 2^118).  Returns the number of violated statements.
+uint64_t hc_class_rule_mismatches(uint64_t samples, uint64_t seed) {
+    static const uint64_t ps[] = {13, 17, 29, 37, 41, 53, 61, 73, 89, 97, 101, 109, 113, 137, 149, 157};
+    constexpr uint32_t NP = sizeof(ps) / sizeof(ps[0]);
+    int64_t pa[NP], pb[NP];
+    for (uint32_t i = 0; i < NP; ++i) prime_split(ps[i], pa[i], pb[i]);
+    uint64_t x = seed | 1;
+    auto next = [&](uint32_t mod) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return (uint32_t)(x % mod);
+    };
+    auto mod = [](i128 v, int64_t q) { return (int64_t)(v % q + (v % q < 0 ? q : 0)); };
     uint64_t bad = 0;
-    for (uint32_t l3 = 0; l3 < 3; ++l3)
-        for (uint32_t l5 = 0; l5 < 5; ++l5)
-            for (uint32_t m3 = 0; m3 < 3; ++m3)
-                for (uint32_t m5 = 0; m5 < 5; ++m5) {
-                    LaneCtx c;
-                    make_lane_ctx(7, 1.0, 1.0, 2.0, l3, l5, m3, m5, c);
-                    const uint32_t h = ((l3 == 0 || m3 == 0) ? 9u : 3u) * ((l5 != 0 && m5 != 0) ? 5u : 1u);
-                    for (uint32_t va = 0; va < 65536; ++va) bad += area_ok(va, c) != (va % (4 * h) == 0);
-                }
-    return bad;
-}
-
-// The device's area products against the area test's definition: for every
-// class h, every single product mul_u24(r << 8, r' * h^-1) must equal
-// (r r' h^-1 mod 2^24) << 8, and `samples` random residue quadruples must give
-// area_ok_x(area_x(...)) == area_ok_s((r1 r1' + r2 r2') << 8) == (4h | va).
-uint64_t hc_area_x_mismatches(uint64_t samples, uint64_t seed) {
-    uint64_t bad = 0;
-    static const uint32_t hs[4] = {3, 9, 15, 45};
-    for (uint32_t hi = 0; hi < 4; ++hi) {
-        const uint32_t h = hs[hi];
-        LaneCtx c;
-        // a context of this class (make_lane_ctx picks h from the residues of l and m mod 3, 5)
-        make_lane_ctx(7, 1.0, 1.0, 2.0, h % 9 == 0 ? 0u : 1u, h % 5 == 0 ? 1u : 0u, h % 9 == 0 ? 0u : 1u,
-                      h % 5 == 0 ? 1u : 0u, c);
-        if (mul_u24(h, c.inv24) % (1u << 24) != 1u) ++bad;  // the class's inverse
-        for (uint32_t r = 0; r < QA; ++r)
-            for (uint32_t rp = 0; rp < QA; ++rp) {
-                const uint32_t want = (uint32_t)(((uint64_t)r * rp * c.inv24) % (1ull << 24)) << 8;
-                bad += mul_u24(r << 8, mul_u24(rp, c.inv24)) != want;
-            }
-        uint64_t x = seed + hi;
-        auto next = [&] {
-            x ^= x << 13;
-            x ^= x >> 7;
-            x ^= x << 17;
-            return (uint32_t)(x % QA);
-        };
-        for (uint64_t t = 0; t < samples; ++t) {
-            const uint32_t r1 = next(), p1 = next(), r2 = next(), p2 = next();
-            const uint32_t va = r1 * p1 + r2 * p2;
-            const bool a = area_ok_x(area_x(r1 << 8, mul_u24(p1, c.inv24), r2 << 8, mul_u24(p2, c.inv24)), c);
-            bad += a != area_ok_s(va << 8, c);
-            bad += a != (va % (4 * h) == 0);
+    for (uint64_t t = 0; t < samples; ++t) {
+        GInt z{1, 0}, w{1, 0};
+        uint32_t cz = 0, cw = 0;
+        const uint32_t nf = 2 + next(2);
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t i = next(NP), E = 1 + next(2), r = next(4 * E + 1);
+            z = gmul(z, gauss_value(pa[i], pb[i], E, r));
+            cz ^= value_class(ps[i], r);
         }
+        for (uint32_t f = 0; f < 2; ++f) {
+            const uint32_t i = next(NP), r = next(5);
+            w = gmul(w, gauss_value(pa[i], pb[i], 1, r));
+            cw ^= value_class(ps[i], r);
+        }
+        bad += mod(z.re, 2) != 1;
+        bad += mod(z.im, 4) != 0;
+        const bool re3 = mod(z.re, 3) == 0, im3 = mod(z.im, 3) == 0;
+        bad += re3 == im3;
+        bad += im3 != ((cz & AREA_C3) == 0);
+        const bool re5 = mod(z.re, 5) == 0, im5 = mod(z.im, 5) == 0;
+        bad += re5 == im5;
+        bad += im5 != ((cz & AREA_C5) == 0);
+        // z * conj(w): class cz ^ cw
+        const GInt y = gmul(z, GInt{w.re, -w.im});
+        const uint32_t cy = cz ^ cw;
+        bad += (mod(y.im, 3) == 0) != ((cy & AREA_C3) == 0);
+        bad += (mod(y.im, 5) == 0) != ((cy & AREA_C5) == 0);
+        bad += (mod(y.re, 3) == 0) == (mod(y.im, 3) == 0);
     }
     return bad;
 }
@@ -494,9 +482,7 @@ uint64_t hc_one_term_violations(uint64_t l, uint64_t limit, uint32_t q, uint32_t
     if (!any && !tile_one_term(q, k0, l4, l2, n4)) return 0;
     uint64_t bad = 0;
     for (uint32_t k = k0; k <= k1; ++k) {
-        LaneCtx c;
-        make_lane_ctx((uint64_t)q * k, l4, l2, n4, 1, 1, 1, 1, c);
-        bad += c.h2_w >= 0.0;
+        bad += make_band((double)q * (double)k, l4, l2, n4).h2_w >= 0.0;
     }
     return bad;
 }

@@ -243,21 +243,15 @@ def test_window_mode_skips_work_near_limit(hc, window_everywhere):
     assert sw.cells < 0.5 * sp.cells, (sw.cells, sp.cells)
 
 
-def test_area_test_is_divisibility(hc):
-    """The fast path's area test (24-bit inverse multiply, rotation) accepts
-    exactly the residues divisible by 4h, h = (3 or 9) * (1 or 5)."""
-    hc.hc_area_mismatches.restype = ctypes.c_uint64
-    assert hc.hc_area_mismatches() == 0
-
-
-def test_area_products(hc):
-    """The kernels fold the multiply by h^-1 into the residue products (area
-    factors r' h^-1 per partner, left residues stored << 8): every single
-    product against (r r' h^-1 mod 2^24) << 8, and sums of two against the
-    plain area test and against divisibility by 4h."""
-    hc.hc_area_x_mismatches.restype = ctypes.c_uint64
-    hc.hc_area_x_mismatches.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-    assert hc.hc_area_x_mismatches(200000, 20261203) == 0
+def test_class_rule(hc):
+    """The class rule the kernels' area tests rest on, on random Gaussian
+    products: Re z odd, 4 | Im z, exactly one part divisible by 3 and by 5,
+    which one given by the xor of the factors' classes (conjugation keeps a
+    factor's class).  A wrong class bit for one prime residue, a wrong parity
+    of r or a product that does not xor fails it."""
+    hc.hc_class_rule_mismatches.restype = ctypes.c_uint64
+    hc.hc_class_rule_mismatches.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    assert hc.hc_class_rule_mismatches(200000, 20261204) == 0
 
 
 def test_device_root_candidate(hc):
