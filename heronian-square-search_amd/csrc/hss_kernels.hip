@@ -458,11 +458,13 @@ __device__ inline void push_global(const CellParams& P, bool pred, const DefItem
 
 // Per-wave LDS ring of leaves with counted terms awaiting the root-proximity
 // test (leaves the FP64 band left undecided go straight to the deferred
-// buffer, defer_leaf).  Entry: X, Y of z/2 and the packed id: (u | partner
-// entry << 17, k | conj << 27 | counted terms << 28), the terms as bits 0-3 for
-// lm - 2|X|, lm + 2|X|, lm - 2|Y|, lm + 2|Y|, never all clear (lm = l^4+m^4).  Slots
-// head .. head + n - 1 (mod QCAP) are live.  One 24-byte entry per slot: one
-// address per push or pop.
+// buffer).  Entry: one row of cell_kernel -- D0 = Re(u w) and D1 = Re(u conj(w))
+// for the leaves u v and u conj(v) in their orientation (w = v or -i v) -- and
+// the packed id (u | partner entry << 17, k | counted terms << 28), the terms as
+// bits 0-3 for lm - 2|D0|, lm + 2|D0|, lm - 2|D1|, lm + 2|D1|, never all clear
+// (lm = l^4+m^4); bits 2-3 belong to the conjugate leaf.  Slots head .. head +
+// n - 1 (mod QCAP) are live.  One 24-byte entry per slot: one address per push
+// or pop.
 struct QEntry {
     double x, y;
     uint32_t lo, hi;
@@ -483,7 +485,6 @@ struct TileCtx {
     double l4_d;     // FP64 l^4
 };
 
-__device__ inline uint32_t pack_ui(uint32_t ui, uint32_t jj) { return ui | jj << ID_U_BITS; }
 
 // One round over the oldest min(n, 64) queued leaves, one per lane: the
 // lowest remaining term is root-tested (a survivor goes to the deferred
@@ -515,7 +516,7 @@ __device__ inline void root_round(const CellParams& P, RootQueue& q, const TileC
     if (__builtin_amdgcn_ballot_w64(pass)) {
         const uint32_t ui = qe.lo & MAX_LEFT_ENTRIES, jj = qe.lo >> ID_U_BITS;
         const uint32_t vidx = pass ? P.off[k] + jj : 0u;
-        push_global(P, pass, DefItem{tc.left, ui, vidx, k | ((qe.hi >> 27) & 1u) << 30});
+        push_global(P, pass, DefItem{tc.left, ui, vidx, k | (t >= 2u ? DEF_CONJ : 0u)});
     }
     const uint32_t rest = terms & (terms - 1u);  // terms left after the lowest
     const uint64_t again = __builtin_amdgcn_ballot_w64(rest != 0u);
@@ -542,40 +543,6 @@ __device__ inline void root_push(const CellParams& P, RootQueue& q, const TileCt
 __device__ inline void root_flush(const CellParams& P, RootQueue& q, const TileCtx& tc) {
     while (q.n) root_round<false>(P, q, tc);
     q.head = 0;  // empty: restart at slot 0 (full rounds rely on head = 0 or 64)
-}
-
-// One leaf z/2 = X + iY of the plain kernel by the class rule (hss_math.h):
-// ok1 / ok2 say whether orientation 1 (d = 2X) / 2 (d = 2Y) passes the area
-// tests, from the classes of u and v and the pair; the FP64 band of each
-// allowed orientation decides its terms.  Returns the counted terms (bits 0-1:
-// term 1, 2 of orientation 1 on |X|; bits 2-3: of orientation 2 on |Y|), the
-// leaf's decided candidates (counted as the root queue tests them); und is set
-// (and no term returned) when a band leaves the leaf to the exact stage.  ONE:
-// the pair has h2_w < 0 (tile_one_term), so term 2 of either orientation is
-// false and its compares are skipped (same result).
-template <bool ONE>
-__device__ inline uint32_t leaf_decide(bool ok1, bool ok2, double X, double Y, const Band& b, bool& und) {
-    bool u1, u2;
-    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
-    const uint32_t f2 = o1_decide<ONE>(ok2, Y, b, u2);
-    und = u1 | u2;
-    return und ? 0u : (f1 | f2 << 2);
-}
-
-// Queue a decided leaf with counted terms (from leaf_decide; kq = k | conj <<
-// 27, idlo = pack_ui(ui, partner entry)) for the root-proximity test.
-__device__ inline void leaf_queue(const CellParams& P, RootQueue& rq, const TileCtx& tc, uint32_t terms, double X,
-                                  double Y, uint32_t kq, uint32_t idlo) {
-    root_push(P, rq, tc, terms != 0u, X, Y, idlo, kq | terms << 28);
-}
-
-// An undecided leaf goes whole to the exact stage (k_exact counts all its
-// terms): its partner entry from (k, jj), marked as a full leaf.  Called by all
-// lanes (push_global is a wave operation), rarely with any und set.
-__device__ inline void defer_leaf(const CellParams& P, const TileCtx& tc, bool und, uint32_t kq, uint32_t idlo) {
-    const uint32_t k = kq & 0x7FFFFFFu;
-    const uint32_t vidx = und ? P.off[k] + (idlo >> ID_U_BITS) : 0u;
-    push_global(P, und, DefItem{tc.left, idlo & MAX_LEFT_ENTRIES, vidx, k | ((kq >> 27) & 1u) << 30 | DEF_FULL});
 }
 
 // One partner entry v (lane data).
@@ -607,9 +574,15 @@ __device__ inline uint32_t pair_area_info(const LeftDesc& L, const PosDesc& S, u
 // half-set entries.
 struct KCtx {
     double mid, half_w, h2_w, tau;
-    uint32_t info;  // pair_area_info
+    uint32_t info;  // pair_area_info | KCTX_TWICE
     uint32_t pad;
 };
+
+// An even pair with no constrained class bit (cm = 0): every leaf passes the
+// area tests in both orientations, so cell_kernel runs its half-set entries
+// twice, once per orientation; every other pair has at most one orientation
+// per leaf.
+static constexpr uint32_t KCTX_TWICE = 8;
 
 __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, uint32_t jj, uint32_t Hj,
                                     uint32_t oj, Partner& v) {
@@ -628,19 +601,23 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
 // pairs with class 3 on cm.  A real v makes the leaf real: orientation 1 has
 // area 0, orientation 2 has d = 0 and counts once when lo <= 0.  Counted terms
 // are root-tested here; an undecided leaf goes whole to the exact stage.
+// force = 1 or 2 restricts the leaf to that orientation (cell_kernel's two
+// passes over the entries of a pair with cm = 0), 0 allows both.
 template <bool ONE>
 __device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
-                                   const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t& cnt) {
+                                   const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t force,
+                                   uint32_t& cnt) {
     const double X = cu * v.c, Y = cu * v.d;
-    const bool ok1 = act && !v.vc && (v.cls & cm) == 0u;
-    const bool ok2 = act && even && ((v.cls ^ 3u) & cm) == 0u;
+    const bool ok1 = act && force != 2u && !v.vc && (v.cls & cm) == 0u;
+    const bool ok2 = act && force != 1u && even && ((v.cls ^ 3u) & cm) == 0u;
     const bool real2 = ok2 && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
     const double h2n = b.h2_w - b.tau;
     bool u1, u2;
     const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
     const uint32_t f2 = o1_decide<ONE>(ok2 && !v.vc, Y, b, u2);
     const bool und = u1 | u2 | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
-    if (__builtin_amdgcn_ballot_w64(und)) push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | DEF_FULL});
+    if (__builtin_amdgcn_ballot_w64(und))
+        push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | (force ? force << 27 : 0u) | DEF_FULL});
     const uint32_t f = und ? 0u : (f1 | f2 << 2);
     cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
     const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
@@ -658,10 +635,11 @@ __device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool 
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
 // (k = kb + 2 lane) are filtered (on_chunk(k) is called with each lane's k
-// first) and their half-set entries flattened; f is called for every 64
-// flattened entries with the lane's entry (act, kj, jj, Hj), the lane j of its
-// k and the offset oj of k's half-set.  pairs and leaves count the pairs that
-// reach the enumeration and their leaves.
+// first and returns how many times k's half-set entries run, 1 or 2) and their
+// half-set entries flattened; f is called for every 64 flattened entries with
+// the lane's entry (act, kj, jj, Hj; with two runs jj < Hj = twice the
+// half-set size), the lane j of its k and the offset oj of k's half-set.  pairs
+// and leaves count the pairs that reach the enumeration and their leaves.
 template <typename C, typename F>
 __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T, const LeftDesc& L, const PosDesc& S,
                                           uint32_t* mark, uint64_t& pairs, uint64_t& leaves, C&& on_chunk, F&& f) {
@@ -680,7 +658,7 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
             pairs += Hk ? 1 : 0;
             // leaves of the pair: 2H - 1 per non-central entry of k, H for its central one
             leaves += Hk ? (uint64_t)Hk * (2u * L.H - 1u) - (L.H - 1u) : 0ull;
-            on_chunk(k);
+            Hk *= on_chunk(k);
             const uint32_t incl = wave_incl_scan(Hk);
             const uint32_t excl = incl - Hk;
             const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
@@ -748,49 +726,70 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         // the pair's band and area info once per k (a k's half-set entries share them)
         const auto on_chunk = [&](uint32_t k) {
             const Band c = pair_band(L, S, k);
+            const uint32_t info = pair_area_info(L, S, k);
+            const bool twice = (info & 3u) == 0u && (info & 4u) != 0u;
             __builtin_amdgcn_wave_barrier();
-            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, pair_area_info(L, S, k), 0u};
+            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, info | (twice ? KCTX_TWICE : 0u), 0u};
             __builtin_amdgcn_wave_barrier();
+            return twice ? 2u : 1u;
         };
         for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
                            [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j, uint32_t oj) {
-            Partner v;
-            load_partner(P, act, kj, jj, Hj, oj, v);
             const KCtx K = lds_at(kctx, j);
+            // a pair run twice: orientation 1 on the first run of its entries, 2 on the second
+            const bool twice = (K.info & KCTX_TWICE) != 0u;
+            const uint32_t Hv = twice ? Hj >> 1 : Hj;
+            const uint32_t force = twice ? (jj < Hv ? 1u : 2u) : 0u;
+            const uint32_t jv = jj < Hv ? jj : jj - Hv;
+            Partner v;
+            load_partner(P, act, kj, jv, Hv, oj, v);
             const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(L.q, kj, S.l4_d)};
             const uint32_t cm = K.info & 3u;
             const bool even = (K.info & 4u) != 0u;
-            const uint32_t jpk = jj << ID_U_BITS;
-            // rows of non-central u: leaves u*v and, unless v is real, u*conj(v)
-            // (same class: conjugation keeps it), instantiated for tiles with
-            // and without second-sign terms
+            // with cm != 0 a leaf passes the area tests in at most one
+            // orientation: 1 when its class is 0 on cm, 2 (even pairs) when 3
+            // on cm.  As compares of c = (class of u v) & cmsk: orientation 1
+            // when c == t1, 2 when c == t2 (a forced run has cmsk = 0 and
+            // matches one of them; 4 never matches).
+            const uint32_t cmsk = force ? 0u : cm;
+            const uint32_t t1 = force == 2u ? 4u : 0u;
+            const uint32_t t2 = force == 2u ? 0u : ((force == 0u && even && cm != 0u) ? cm : 4u);
+            const uint32_t jpk = jv << ID_U_BITS;
+            // rows of non-central u: the leaves u v and, unless v is real,
+            // u conj(v) (same class: conjugation keeps it), both in the
+            // orientation of their class.  Orientation 2 of a leaf is
+            // orientation 1 of the leaf times -i, so each row evaluates
+            // D = Re(u w) and Re(u conj(w)) with w = v or -i v; instantiated for
+            // tiles with and without second-sign terms
             const auto rows = [&](auto one_c) {
                 constexpr bool ONE = decltype(one_c)::value;
                 for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
                     const LeftFast u = lf[ui];
-                    const uint32_t c = (u.cls ^ v.cls) & cm;
-                    const bool ok1 = act && c == 0u, ok2 = act && even && c == cm;
-                    const double t1 = u.im * v.d, t2 = u.re * v.d;
-                    const double X0 = __builtin_fma(u.re, v.c, -t1), Y0 = __builtin_fma(u.im, v.c, t2);
-                    const double X1 = __builtin_fma(u.re, v.c, t1), Y1 = __builtin_fma(u.im, v.c, -t2);
+                    const uint32_t c = (u.cls ^ v.cls) & cmsk;
+                    const bool is2 = c == t2;
+                    const bool live = act && (c == t1 || is2);
+                    const double ce = is2 ? v.d : v.c, de = is2 ? -v.c : v.d;
+                    const double t = u.im * de;
+                    const double D0 = __builtin_fma(u.re, ce, -t), D1 = __builtin_fma(u.re, ce, t);
                     bool u0, u1;
-                    const uint32_t f0 = leaf_decide<ONE>(ok1, ok2, X0, Y0, b, u0);
-                    const uint32_t f1 = leaf_decide<ONE>(ok1 & !v.vc, ok2 & !v.vc, X1, Y1, b, u1);
-                    if (__builtin_amdgcn_ballot_w64(u0 | u1)) {  // rare
-                        defer_leaf(P, tc, u0, kj, ui | jpk);
-                        defer_leaf(P, tc, u1, kj | 1u << 27, ui | jpk);
+                    const uint32_t f0 = o1_decide<ONE>(live, D0, b, u0);
+                    const uint32_t f1 = o1_decide<ONE>(live && !v.vc, D1, b, u1);
+                    if (__builtin_amdgcn_ballot_w64(u0 | u1)) {  // rare: the leaf in its orientation only
+                        const uint32_t ko = kj | (is2 ? DEF_O2 : DEF_O1) | DEF_FULL;
+                        push_global(P, u0, DefItem{tc.left, ui, v.vidx, ko});
+                        push_global(P, u1, DefItem{tc.left, ui, v.vidx, ko | DEF_CONJ});
                     }
-                    leaf_queue(P, rq, tc, f0, X0, Y0, kj, ui | jpk);
-                    leaf_queue(P, rq, tc, f1, X1, Y1, kj | 1u << 27, ui | jpk);
+                    const uint32_t f = f0 | f1 << 2;
+                    root_push(P, rq, tc, f != 0u, D0, D1, ui | jpk, kj | f << 28);
                 }
             };
             const double cu = lf[L.H - 1].re;  // the central entry is real
             if (one) {
                 rows(std::true_type{});
-                central_win<true>(P, tc, act, cu, L.H - 1, v, b, cm, even, cnt);
+                central_win<true>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
             } else {
                 rows(std::false_type{});
-                central_win<false>(P, tc, act, cu, L.H - 1, v, b, cm, even, cnt);
+                central_win<false>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
             }
         });
         root_flush(P, rq, tc);
@@ -956,7 +955,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [](uint32_t) {},
+        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [](uint32_t) { return 1u; },
                            [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t, uint32_t oj) {
             // ---- phase 1: lane = partner entry
             Partner v;
@@ -966,9 +965,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             const uint32_t cm = ainfo & 3u;
             const bool even = (ainfo & 4u) != 0u;
             if (one)
-                central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, cnt);
+                central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, 0u, cnt);
             else
-                central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, cnt);
+                central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, 0u, cnt);
             evald += act ? 1u : 0u;
             const double md = (double)L.q * (double)kj;  // exact: m < 2^26
             const double m2 = md * md;                      // exact: m^2 < 2^52

@@ -278,19 +278,25 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                 const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
                 const uint32_t cm = area_constrained(need9, need5);
                 const bool even = D.l_even || !(k & 1u);
+                // cell_kernel runs the entries of an even pair with cm = 0 twice,
+                // orientation 1 (force 1) then 2 (force 2)
+                const bool twice = mode == 0 && cm == 0u && even;
+                for (uint32_t run = 0; run < (twice ? 2u : 1u); ++run)
                 for (uint32_t jj = 0; jj < Hk; ++jj) {
+                    const uint32_t force = twice ? run + 1 : 0u;
                     const uint32_t vidx = T.off[k] + jj;
                     const bool vc = jj + 1 == Hk;
                     const double c = T.cre[vidx], d = T.cim[vidx];
                     const uint32_t vcls = (T.cres[vidx] >> CRES_CLASS_SHIFT) & 3u;
-                    leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
+                    if (run == 0) leaves[D.out] += vc ? L.H : 2ull * L.H - 1;
                     {
                         // central_win: the central u by the class rule (both kernels)
                         ++S.cells;
-                        check_class(L, D, L.H - 1, vidx, k, 0, vc);
+                        if (run == 0) check_class(L, D, L.H - 1, vidx, k, 0, vc);
                         const double cu = lfast[L.off + L.H - 1].re;
                         const double X = cu * c, Y = cu * d;
-                        const bool ok1 = !vc && (vcls & cm) == 0u, ok2 = even && ((vcls ^ 3u) & cm) == 0u;
+                        const bool ok1 = force != 2u && !vc && (vcls & cm) == 0u;
+                        const bool ok2 = force != 1u && even && ((vcls ^ 3u) & cm) == 0u;
                         const bool real2 = ok2 && vc;
                         const double h2n = bd.h2_w - bd.tau;
                         bool u1, u2;
@@ -299,7 +305,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                                                 : o1_decide<false>(ok2 && !vc, Y, bd, u2);
                         const bool und = u1 || u2 || (real2 && bd.h2_w >= 0.0 && h2n < 0.0);
                         if (und) {
-                            defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 1, 3});
+                            defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 1, force ? force : 3u});
                             ++S.deferred_full;
                         } else {
                             const uint32_t f = f1 | f2 << 2;
@@ -308,28 +314,35 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                         }
                     }
                     if (mode == 0) {
-                        // cell_kernel rows: every non-central u, leaves u*v and (v not real) u*conj(v)
-                        for (uint32_t ui = 0; ui + 1 < L.H; ++ui)
+                        // cell_kernel rows: every non-central u, leaves u*v and (v not real)
+                        // u*conj(v), each in the one orientation its class allows (or the
+                        // run's forced one): D = Re(u w), Re(u conj(w)) with w = v or -i v
+                        const uint32_t cmsk = force ? 0u : cm;
+                        const uint32_t t1 = force == 2u ? 4u : 0u;
+                        const uint32_t t2 = force == 2u ? 0u : ((force == 0u && even && cm != 0u) ? cm : 4u);
+                        for (uint32_t ui = 0; ui + 1 < L.H; ++ui) {
+                            const LeftFast& u = lfast[L.off + ui];
+                            const uint32_t cr = (u.cls ^ vcls) & cmsk;
+                            const bool is2 = cr == t2, live = cr == t1 || is2;
+                            if (run == 0)
+                                for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) check_class(L, D, ui, vidx, k, cj, false);
+                            const double ce = is2 ? d : c, de = is2 ? -c : d;
+                            const double tt = u.im * de;
                             for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
                                 ++S.cells;
-                                check_class(L, D, ui, vidx, k, cj, false);
-                                const LeftFast& u = lfast[L.off + ui];
-                                const uint32_t cr = (u.cls ^ vcls) & cm;
-                                const bool ok1 = cr == 0u, ok2 = even && cr == cm;
-                                const double t1 = u.im * d, t2 = u.re * d;
-                                const double X = cj ? __builtin_fma(u.re, c, t1) : __builtin_fma(u.re, c, -t1);
-                                const double Y = cj ? __builtin_fma(u.im, c, -t2) : __builtin_fma(u.im, c, t2);
-                                bool u1, u2;
-                                const uint32_t f1 = one ? o1_decide<true>(ok1, X, bd, u1) : o1_decide<false>(ok1, X, bd, u1);
-                                const uint32_t f2 = one ? o1_decide<true>(ok2, Y, bd, u2) : o1_decide<false>(ok2, Y, bd, u2);
-                                if (u1 || u2) {
-                                    defs.push_back({Tl.left, ui, vidx, k, cj, 1, 3});
+                                if (!live) continue;
+                                const double Dw = cj ? __builtin_fma(u.re, ce, tt) : __builtin_fma(u.re, ce, -tt);
+                                bool und;
+                                const uint32_t f = one ? o1_decide<true>(true, Dw, bd, und) : o1_decide<false>(true, Dw, bd, und);
+                                if (und) {
+                                    defs.push_back({Tl.left, ui, vidx, k, cj, 1, is2 ? 2u : 1u});
                                     ++S.deferred_full;
                                     continue;
                                 }
-                                cand[D.out] += __builtin_popcount(f1 | f2 << 2);
-                                root_test(Tl.left, ui, vidx, k, cj, f1 | f2 << 2, X, Y, bd.lm);
+                                cand[D.out] += __builtin_popcount(f);
+                                root_test(Tl.left, ui, vidx, k, cj, f, Dw, 0.0, bd.lm);
                             }
+                        }
                         continue;
                     }
                     // cell_win_kernel: passes over (class variant j, orientation rho)
