@@ -13,16 +13,22 @@ candidate count is checked against the first step's, and the slice's total
 against the reference's 39,091,522,776 (bit-exact or the run fails).
 
 Prints one JSON line.  roofline: the enumeration runs as two persistent
-kernels, cell_win_kernel (tiles whose range window is narrow: only leaves that
-can pass the range test are evaluated) and cell_kernel (every leaf); the line
-describes the one with the longer HIP-event duration.  Both are bound by the
-vector ALU (integer and FP64 lane operations), not by HBM or MFMA; achieved =
-64 lane-ops per evaluated leaf (the fast path's essential arithmetic) times
-the kernel's evaluated leaves per launch, divided by its duration; peak = 256
-CU x 64 lanes x 2.4 GHz.
-roofline.enumerated_ops: the same kernel time charged with every leaf of the
-tiles it ran (64 lane-ops per enumerated leaf plus 32 per candidate), the
-convention that counts the leaves the range windows skip as done.
+kernels, cell_win_kernel (tiles whose range window is narrow: only leaves of
+the one area class and orientation that can count, inside the range window,
+are evaluated) and cell_kernel (every leaf in the one orientation its area
+class allows); the line describes the one with the longer HIP-event duration.
+Both are bound by the vector ALU (FP64 and integer lane operations), not by
+HBM or MFMA.  achieved = (8 lane-ops per evaluated leaf orientation + 16 per
+candidate) / the kernel's duration, with its own evaluated and candidate
+counts: 8 = the FP64 coordinate D = Re(u w) (mul + fma), |D| - mid and four
+band compares, the count; 16 = zz = l^4 + m^4 -+ 2|D|, two reciprocal square
+roots, the Newton step, rint and |n^4 - zz| <= 2^-47 zz.  peak = 256 CU x 64
+lanes x 2.4 GHz.  The rest of a kernel's instructions (partner and tile
+bookkeeping, queueing, masked lanes) is overhead this fraction exposes.
+roofline.enumerated_ops: both kernels' time charged with every leaf of the
+enumeration at 64 lane-ops per leaf plus 32 per candidate (the exact-product
+convention, in which skipped leaves count as done; above 1 means the skipping
+beats that roofline).
 cpu_baseline: the reference search_v18 compiled from source (oracle/_ref/search),
 timed on this host's cores (one single-position process per core, all at
 once) on a bounded, seeded sample of the batch's roots from each of its three
@@ -67,7 +73,9 @@ DENSE_HITS = [(1853, 4380, 4427, 32918611718880), (11789, 68104, 68595, 28423956
 DENSE_REF_SECONDS, DENSE_REF_THREADS = 1083.5, 8  # that run's wall time on the fixture host
 GOLDEN = ROOT / "tests" / "golden" / "reference_cases.jsonl"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # lane-ops per second, in T
-OPS_PER_LEAF = 64  # lane-ops of one evaluated leaf's fast-path arithmetic
+OPS_PER_EVAL = 8   # lane-ops of one evaluated leaf orientation (FP64 coordinate and band test)
+OPS_PER_CAND = 16  # lane-ops of one candidate's fourth-root proximity test
+OPS_PER_LEAF = 64  # the exact-product convention: one enumerated leaf
 
 
 def profiled_traffic(kernel: str, evaluated: int) -> int | None:
@@ -393,9 +401,13 @@ def main() -> int:
     w_ms, p_ms = sum(win_ms) / len(win_ms), sum(plain_ms) / len(plain_ms)
     w_eval = stats["window_evaluated"]
     p_eval = stats["evaluated"] - w_eval
-    kernel, k_ms, k_eval = ("cell_win_kernel", w_ms, w_eval) if w_ms >= p_ms else ("cell_kernel", p_ms, p_eval)
-    ops = OPS_PER_LEAF * k_eval
+    w_cand, p_cand = stats["window_candidates"], stats["plain_candidates"]
+    kernel, k_ms, k_eval, k_cand = (("cell_win_kernel", w_ms, w_eval, w_cand) if w_ms >= p_ms
+                                    else ("cell_kernel", p_ms, p_eval, p_cand))
+    ops = OPS_PER_EVAL * k_eval + OPS_PER_CAND * k_cand
     achieved = ops / (k_ms / 1e3) / 1e12
+    other_ops = OPS_PER_EVAL * (w_eval + p_eval - k_eval) + OPS_PER_CAND * (w_cand + p_cand - k_cand)
+    other_ms = p_ms if kernel == "cell_win_kernel" else w_ms
     # the enumerated-leaf convention: every leaf of the whole enumeration, and
     # every candidate, charged over both cell kernels' time
     enum_ops = OPS_PER_LEAF * leaves + 32 * cands
@@ -418,12 +430,15 @@ def main() -> int:
                    "limit": LIMIT, "positions": len(roots), "parallelism": "1 GPU, persistent grid"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": profiled_traffic(kernel, k_eval),
-                     "kernel": kernel, "kernel_ms": k_ms, "evaluated_leaves": k_eval, "ops_per_leaf": OPS_PER_LEAF,
-                     "ops_per_launch": ops, "enumerated_ops": enum_ops, "enumerated_achieved": enum_achieved,
+                     "kernel": kernel, "kernel_ms": k_ms, "evaluated": k_eval, "candidates": k_cand,
+                     "ops_per_eval": OPS_PER_EVAL, "ops_per_candidate": OPS_PER_CAND, "ops_per_launch": ops,
+                     "other_kernel_frac": other_ops / (other_ms / 1e3) / 1e12 / VALU_PEAK_TOPS,
+                     "enumerated_ops": enum_ops, "enumerated_achieved": enum_achieved,
                      "enumerated_frac": enum_achieved / VALU_PEAK_TOPS, "enumerated_over_s": cell_s},
         "work": {"leaves": leaves, "evaluated": stats["evaluated"], "candidates": cands, "tiles": stats["tiles"],
                  "window_tiles": stats["window_tiles"], "window_ms": w_ms, "window_evaluated": w_eval,
-                 "plain_ms": p_ms, "plain_evaluated": p_eval, "deferred": stats["deferred"],
+                 "plain_ms": p_ms, "plain_evaluated": p_eval, "window_candidates": w_cand,
+                 "plain_candidates": p_cand, "deferred": stats["deferred"],
                  "setup_ms": stats["setup_ms"], "plan_ms": stats["plan_ms"],
                  "exact_ms": sum(exact_ms) / len(exact_ms), "leaves_per_s": leaves / cell_s},
         "per_root_fixtures_checked": per_root_checked,

@@ -63,7 +63,7 @@ struct LaunchBuffers {
     DevBuf header, pos, lefts, builds, tiles, wtiles, sorted, perm, btab, lfast, lre, lim, misc, def, raw, hits;
     HostBuf upload;   // pinned staging of the plan
     HostBuf upload2;  // pinned staging of the tiles when they are uploaded after the setup stage
-    HostBuf h_misc;   // pinned copy of misc: [1] deferred, [2] raw, [3] hits, [6..9] evaluated (2 x u64)
+    HostBuf h_misc;   // pinned copy of misc: [1] deferred, [2] raw, [3] hits, [6..13] per-kernel totals (4 x u64)
     uint32_t def_cap = 0;
     static constexpr int STAGES = 4;
     // start, setup done, window cells done, plain cells done, exact done, misc
@@ -94,6 +94,7 @@ struct LaunchResult {
     bool graph = false;        // the kernel chain ran as a graph replay
     uint64_t evaluated = 0;    // leaves evaluated by the fast path
     uint64_t win_evaluated = 0, plain_evaluated = 0;
+    uint64_t win_cand = 0, plain_cand = 0;  // candidates counted by each cell kernel
     float setup_ms = 0, cells_ms = 0, exact_ms = 0;
     float win_ms = 0, plain_ms = 0;  // the two parts of cells_ms
 };

@@ -36,7 +36,9 @@ struct CellParams {
     unsigned long long* cand;    // per counter index (PosDesc::out)
     unsigned long long* leaves;  // per counter index
     unsigned long long* pairs;   // per counter index: pairs reaching the enumeration
-    unsigned long long* evaluated;  // leaves evaluated by the fast path: [0] window kernel, [1] plain kernel
+    // per kernel ([0] + i: window kernel, [1] + i: plain kernel): [i = 0] leaf
+    // orientations the fast path tested, [i = 2] candidates it counted
+    unsigned long long* evaluated;
     DefItem* def;     // leaves for the exact kernel
     uint32_t* ndef;
     uint32_t def_cap;

@@ -596,41 +596,38 @@ __device__ inline void load_partner(const CellParams& P, bool act, uint32_t kj, 
 }
 
 // The leaf c_u * v of the left set's central (real) entry with partner entry
-// v by the class rule: orientation 1 (d = c_u Re v) counts only when v's class
-// is 0 on the constrained bits cm, orientation 2 (d = c_u Im v) only on even
-// pairs with class 3 on cm.  A real v makes the leaf real: orientation 1 has
-// area 0, orientation 2 has d = 0 and counts once when lo <= 0.  Counted terms
-// are root-tested here; an undecided leaf goes whole to the exact stage.
-// force = 1 or 2 restricts the leaf to that orientation (cell_kernel's two
-// passes over the entries of a pair with cm = 0), 0 allows both.
+// v, in its one orientation o: by the class rule orientation 1 (d = c_u Re v)
+// counts only when v's class is 0 on the constrained bits cm, orientation 2
+// (d = c_u Im v) only on even pairs with class 3 on cm.  With cm != 0 at most
+// one of them holds; an even pair with cm = 0 runs its entries twice, force =
+// 1 and 2 naming the orientation of each run (force = 0: by class).  A real v
+// makes the leaf real: orientation 1 has area 0, orientation 2 has d = 0 and
+// counts once when lo <= 0.  Counted terms are root-tested here; an undecided
+// leaf goes to the exact stage with its orientation.  Returns whether the leaf
+// had an orientation to test.
 template <bool ONE>
-__device__ inline void central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
+__device__ inline bool central_win(const CellParams& P, const TileCtx& tc, bool act, double cu, uint32_t ui,
                                    const Partner& v, const Band& b, uint32_t cm, bool even, uint32_t force,
                                    uint32_t& cnt) {
-    const double X = cu * v.c, Y = cu * v.d;
-    const bool ok1 = act && force != 2u && !v.vc && (v.cls & cm) == 0u;
-    const bool ok2 = act && force != 1u && even && ((v.cls ^ 3u) & cm) == 0u;
-    const bool real2 = ok2 && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
+    const uint32_t c = v.cls & cm;
+    const uint32_t o = force ? force : (c == 0u ? 1u : ((even && c == cm) ? 2u : 0u));
+    const bool ok = act && o != 0u;
+    const double D = cu * (o == 2u ? v.d : v.c);
+    const bool real2 = ok && o == 2u && v.vc;  // -lo - tau >= 0 <=> lo <= -tau: decided; |lo| <= tau: undecided
     const double h2n = b.h2_w - b.tau;
-    bool u1, u2;
-    const uint32_t f1 = o1_decide<ONE>(ok1, X, b, u1);
-    const uint32_t f2 = o1_decide<ONE>(ok2 && !v.vc, Y, b, u2);
-    const bool und = u1 | u2 | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
-    if (__builtin_amdgcn_ballot_w64(und))
-        push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | (force ? force << 27 : 0u) | DEF_FULL});
-    const uint32_t f = und ? 0u : (f1 | f2 << 2);
+    bool u;
+    const uint32_t f1 = o1_decide<ONE>(ok && !v.vc, D, b, u);
+    const bool und = u | (real2 && b.h2_w >= 0.0 && h2n < 0.0);
+    if (__builtin_amdgcn_ballot_w64(und)) push_global(P, und, DefItem{tc.left, ui, v.vidx, v.kj | o << 27 | DEF_FULL});
+    const uint32_t f = und ? 0u : f1;
     cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
-    const double ax = __builtin_fabs(X), ay = __builtin_fabs(Y);
+    const double a = __builtin_fabs(D);
     bool pass = false;
-#pragma unroll
-    for (uint32_t t = 0; t < 4; ++t) {
-        if (ONE && (t & 1u)) continue;
-        const bool on = (f >> t) & 1u;
-        if (!__builtin_amdgcn_ballot_w64(on)) continue;
-        const double a = t < 2u ? ax : ay;
-        pass |= on && root_near_integer(__builtin_fma((t & 1u) ? 2.0 : -2.0, a, b.lm));
-    }
+    if (__builtin_amdgcn_ballot_w64(f & 1u)) pass = (f & 1u) && root_near_integer(__builtin_fma(-2.0, a, b.lm));
+    if (!ONE && __builtin_amdgcn_ballot_w64(f & 2u))
+        pass |= (f & 2u) && root_near_integer(__builtin_fma(2.0, a, b.lm));
     if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
+    return (ok && !v.vc) || real2;  // the leaf had an orientation to test
 }
 
 // Partner chunks of a tile: per parity of k, 64 consecutive k of that parity
@@ -679,10 +676,12 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
     }
 }
 
-// Tile bookkeeping shared by both cell kernels: per-position counters.
-// cand = the lanes' direct counts cnt plus the queue's tested terms.
+// Tile bookkeeping shared by both cell kernels: per-position counters and the
+// kernel's own totals (kstat[0]: evaluated leaf orientations, kstat[2]:
+// candidates).  cand = the lanes' direct counts cnt plus the queue's tested
+// terms.
 __device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t cnt, uint64_t tested,
-                                 uint64_t leaves, uint64_t pairs, uint64_t evald, unsigned long long* evaluated) {
+                                 uint64_t leaves, uint64_t pairs, uint64_t evald, unsigned long long* kstat) {
     const uint32_t lane = __lane_id();
     const uint64_t cand = wave_sum64((uint64_t)cnt) + tested;
     leaves = wave_sum64(leaves);
@@ -692,7 +691,8 @@ __device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t
     atomicAdd(P.cand + S.out, lane == 0 ? (unsigned long long)cand : 0ull);
     atomicAdd(P.leaves + S.out, lane == 0 ? (unsigned long long)leaves : 0ull);
     atomicAdd(P.pairs + S.out, lane == 0 ? (unsigned long long)pairs : 0ull);
-    atomicAdd(evaluated, lane == 0 ? (unsigned long long)evald : 0ull);
+    atomicAdd(kstat, lane == 0 ? (unsigned long long)evald : 0ull);
+    atomicAdd(kstat + 2, lane == 0 ? (unsigned long long)cand : 0ull);
 }
 
 // The batch arrays the cell kernels only read, as separate restrict-qualified
@@ -707,10 +707,12 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
     __shared__ QEntry s_q[WAVES][QCAP];
     __shared__ uint32_t s_mark[WAVES][65];
     __shared__ KCtx s_kctx[WAVES][64];
+    __shared__ uint32_t s_ncls[WAVES][16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     RootQueue rq{s_q[wave], 0, 0, 0};
     uint32_t* mark = s_mark[wave];
     KCtx* kctx = s_kctx[wave];
+    uint32_t* ncls = s_ncls[wave];
     for (;;) {
         // next tile: all lanes execute the atomic (only lane 0 adds 1), see push_global
         const uint32_t t = __builtin_amdgcn_readfirstlane(atomicAdd(P.next_tile, lane == 0 ? 1u : 0u));
@@ -722,7 +724,28 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const LeftFast* lf = lfast + L.off;
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
-        uint64_t leaves = 0, pairs = 0;
+        uint64_t leaves = 0, pairs = 0, evald = 0;
+        // for the count of leaf orientations the rows test: ncls[x + 4 msk] = the
+        // non-central entries u with (class of u ^ x) & msk == 0
+        if (L.H > 1) {
+            uint64_t h01 = 0, h23 = 0;  // class counts, 32 bits each
+            for (uint32_t i = (uint32_t)lane; i + 1 < L.H; i += 64) {
+                const uint32_t c = lf[i].cls;
+                h01 += c == 0u ? 1ull : (c == 1u ? 1ull << 32 : 0ull);
+                h23 += c == 2u ? 1ull : (c == 3u ? 1ull << 32 : 0ull);
+            }
+            h01 = wave_sum64(h01);
+            h23 = wave_sum64(h23);
+            const uint32_t x = (uint32_t)lane & 3u, msk = ((uint32_t)lane >> 2) & 3u;
+            uint32_t n = 0;
+            n += (x & msk) == 0u ? (uint32_t)h01 : 0u;
+            n += ((1u ^ x) & msk) == 0u ? (uint32_t)(h01 >> 32) : 0u;
+            n += ((2u ^ x) & msk) == 0u ? (uint32_t)h23 : 0u;
+            n += ((3u ^ x) & msk) == 0u ? (uint32_t)(h23 >> 32) : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 16) ncls[lane] = n;
+            __builtin_amdgcn_wave_barrier();
+        }
         // the pair's band and area info once per k (a k's half-set entries share them)
         const auto on_chunk = [&](uint32_t k) {
             const Band c = pair_band(L, S, k);
@@ -784,16 +807,24 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 }
             };
             const double cu = lf[L.H - 1].re;  // the central entry is real
+            // leaf orientations the rows test: entries matching t1, then t2
+            if (L.H > 1 && act) {
+                const uint32_t nrow = (t1 == 0u ? ncls[v.cls + 4u * cmsk] : 0u) +
+                                      (t2 != 4u ? ncls[(v.cls ^ 3u) + 4u * cmsk] : 0u);
+                evald += (uint64_t)nrow << (v.vc ? 0 : 1);
+            }
+            bool cev;
             if (one) {
                 rows(std::true_type{});
-                central_win<true>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
+                cev = central_win<true>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
             } else {
                 rows(std::false_type{});
-                central_win<false>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
+                cev = central_win<false>(P, tc, act, cu, L.H - 1, v, b, cm, even, force, cnt);
             }
+            evald += cev ? 1u : 0u;
         });
         root_flush(P, rq, tc);
-        tile_done(P, S, cnt, rq.tested, leaves, pairs, leaves, P.evaluated + 1);  // every leaf is evaluated
+        tile_done(P, S, cnt, rq.tested, leaves, pairs, evald, P.evaluated + 1);
         rq.tested = 0;
     }
 }
@@ -829,7 +860,7 @@ struct alignas(16) WinPart {
     double thv;                    // angle of v modulo pi
     uint32_t vidx;                 // v's cache entry
     uint32_t kj;                   // k
-    uint32_t info;                 // class | cm << 2 | even << 4 | real v << 5 | window arcs << 6
+    uint32_t info;                 // class | cm << 2 | even << 4 | real v << 5 | window arcs << 6 | force << 8
     uint32_t pad;
 };
 static_assert(sizeof(WinPart) == 96, "LDS budget of cell_win_kernel");
@@ -955,25 +986,33 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [](uint32_t) { return 1u; },
+        // an even pair with cm = 0 runs its entries twice, one orientation per run
+        const auto twice_of = [&](uint32_t k) {
+            const uint32_t info = pair_area_info(L, S, k);
+            return (info & 3u) == 0u && (info & 4u) != 0u;
+        };
+        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [&](uint32_t k) { return twice_of(k) ? 2u : 1u; },
                            [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t, uint32_t oj) {
             // ---- phase 1: lane = partner entry
-            Partner v;
-            load_partner(P, act, kj, jj, Hj, oj, v);
-            const Band b = pair_band(L, S, kj);
             const uint32_t ainfo = pair_area_info(L, S, kj);
             const uint32_t cm = ainfo & 3u;
             const bool even = (ainfo & 4u) != 0u;
-            if (one)
-                central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, 0u, cnt);
-            else
-                central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, 0u, cnt);
-            evald += act ? 1u : 0u;
+            const bool twice = cm == 0u && even;
+            const uint32_t Hv = twice ? Hj >> 1 : Hj;
+            const uint32_t force = twice ? (jj < Hv ? 1u : 2u) : 0u;
+            Partner v;
+            load_partner(P, act, kj, jj < Hv ? jj : jj - Hv, Hv, oj, v);
+            const Band b = pair_band(L, S, kj);
+            evald += (one ? central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt)
+                          : central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt))
+                         ? 1u
+                         : 0u;
             const double md = (double)L.q * (double)kj;  // exact: m < 2^26
             const double m2 = md * md;                      // exact: m^2 < 2^52
             Window1 w;
             make_window_o1((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, b.tau, 2.0 * S.l2_d * m2, w);
-            const uint32_t nvp = act ? allowed_count(cm, kmask) << (even ? 1 : 0) : 0u;
+            // virtual partners: class variants j times the orientations of this run
+            const uint32_t nvp = act ? allowed_count(cm, kmask) << ((even && !force) ? 1 : 0) : 0u;
             __builtin_amdgcn_wave_barrier();
             {
                 WinPart& R = part[lane];
@@ -989,7 +1028,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 R.thv = theta_of_f(v.c, v.d);
                 R.vidx = v.vidx;
                 R.kj = kj;
-                R.info = v.cls | cm << 2 | (even ? 16u : 0u) | (v.vc ? 32u : 0u) | (uint32_t)(w.n & 3) << 6;
+                R.info = v.cls | cm << 2 | (even ? 16u : 0u) | (v.vc ? 32u : 0u) | (uint32_t)(w.n & 3) << 6 | force << 8;
                 R.pad = 0;
             }
             const uint32_t vincl = wave_incl_scan(nvp);
@@ -1009,7 +1048,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const uint32_t info = V.info;
                 const uint32_t vcls = info & 3u, vcm = (info >> 2) & 3u;
                 const bool veven = (info >> 4) & 1u, vreal = (info >> 5) & 1u;
-                const uint32_t rho = veven ? (q & 1u) : 0u, j = veven ? q >> 1 : q;
+                const uint32_t vforce = (info >> 8) & 3u;
+                const bool both = veven && !vforce;
+                const uint32_t rho = vforce ? vforce - 1u : (both ? (q & 1u) : 0u), j = both ? q >> 1 : q;
                 const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, j), kmask);
                 const uint32_t* btg = bt + g * nbg;
                 Window1 vw;

@@ -610,7 +610,8 @@ static int upload_cells(LaunchBuffers& B, HostBuf& up, bool with_header, hipStre
 }
 
 static CellParams cell_params(hss_ctx* c, LaunchBuffers& B, const Counters& cnt) {
-    // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile, [6..9] evaluated
+    // misc: [0] next plain tile, [1] deferred, [2] raw, [3] hits, [4] next window tile,
+    // [6..9] evaluated (2 x u64), [10..13] candidates per kernel (2 x u64)
     uint32_t* misc = B.misc.as<uint32_t>();
     CellParams K{};
     K.off = c->d_off;
@@ -676,7 +677,7 @@ static int run_stages(hss_ctx* c, LaunchBuffers& B, const CellParams& K, const C
     }
     B.replayed = replay;
     vlog(st, "kernel chain");
-    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(B.ev[5], st));
     return HSS_OK;
 }
@@ -719,6 +720,8 @@ void launch_read_misc(const LaunchBuffers& B, LaunchResult& out) {
     out.raw = m4[2];
     std::memcpy(&out.win_evaluated, m4 + 6, 8);
     std::memcpy(&out.plain_evaluated, m4 + 8, 8);
+    std::memcpy(&out.win_cand, m4 + 10, 8);
+    std::memcpy(&out.plain_cand, m4 + 12, 8);
     out.evaluated = out.win_evaluated + out.plain_evaluated;
     out.graph = B.replayed;
     (void)hipEventElapsedTime(&out.setup_ms, B.ev[0], B.ev[1]);
@@ -822,6 +825,8 @@ static int finish_search(hss_ctx* c, const LaunchBuffers& B, const LaunchResult&
     c->stats.evaluated = R.evaluated;
     c->stats.graph_launch = R.graph ? 1 : 0;
     c->stats.window_evaluated = R.win_evaluated;
+    c->stats.window_candidates = R.win_cand;
+    c->stats.plain_candidates = R.plain_cand;
     c->stats.window_ms = R.win_ms;
     c->stats.plain_ms = R.plain_ms;
     c->stats.sorted_lefts = B.plan.sorted.size();
@@ -971,7 +976,7 @@ extern "C" int hss_search_noalloc(hss_ctx* c, uint64_t limit, const uint64_t* ro
         if (stage == 0) HIP_TRY(hipEventRecord(B.ev[6], st));
     }
     B.replayed = false;
-    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(B.h_misc.p, B.misc.p, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(B.ev[5], st));
     HIP_TRY(hipEventSynchronize(B.ev[5]));
     LaunchResult R;

@@ -42,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("block_threads", ctypes.c_uint32), ("window_tiles", ctypes.c_uint64),
                 ("sorted_lefts", ctypes.c_uint64), ("evaluated", ctypes.c_uint64), ("window_evaluated", ctypes.c_uint64),
                 ("window_ms", ctypes.c_double),
-                ("plain_ms", ctypes.c_double), ("graph_launch", ctypes.c_uint64)]
+                ("plain_ms", ctypes.c_double), ("graph_launch", ctypes.c_uint64),
+                ("window_candidates", ctypes.c_uint64), ("plain_candidates", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

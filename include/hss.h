@@ -65,11 +65,14 @@ typedef struct {
     uint32_t block_threads;
     uint64_t window_tiles; /* tiles run by the window-mode kernel (range windows) */
     uint64_t sorted_lefts; /* left sets stored sorted by angle for it */
-    uint64_t evaluated;    /* leaves evaluated by the fast path (leaves minus those skipped) */
+    uint64_t evaluated;    /* leaf orientations the fast path tested (the class rule and the
+                              range windows skip the rest) */
     uint64_t window_evaluated; /* ... of them by the window-mode kernel */
     double window_ms;      /* the window-mode kernel's part of cells_ms */
     double plain_ms;       /* the plain kernel's part of cells_ms */
     uint64_t graph_launch; /* 1 when the kernel chain ran as a replay of its captured hipGraph */
+    uint64_t window_candidates; /* candidates counted by the window-mode kernel */
+    uint64_t plain_candidates;  /* ... by the plain kernel (the exact stage counts the rest) */
 } hss_stats;
 
 /*

@@ -278,9 +278,9 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                 const bool need9 = D.l3 == 0 || m % 3 == 0, need5 = D.l5 != 0 && m % 5 != 0;
                 const uint32_t cm = area_constrained(need9, need5);
                 const bool even = D.l_even || !(k & 1u);
-                // cell_kernel runs the entries of an even pair with cm = 0 twice,
+                // both kernels run the entries of an even pair with cm = 0 twice,
                 // orientation 1 (force 1) then 2 (force 2)
-                const bool twice = mode == 0 && cm == 0u && even;
+                const bool twice = cm == 0u && even;
                 for (uint32_t run = 0; run < (twice ? 2u : 1u); ++run)
                 for (uint32_t jj = 0; jj < Hk; ++jj) {
                     const uint32_t force = twice ? run + 1 : 0u;
@@ -294,23 +294,21 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                         ++S.cells;
                         if (run == 0) check_class(L, D, L.H - 1, vidx, k, 0, vc);
                         const double cu = lfast[L.off + L.H - 1].re;
-                        const double X = cu * c, Y = cu * d;
-                        const bool ok1 = force != 2u && !vc && (vcls & cm) == 0u;
-                        const bool ok2 = force != 1u && even && ((vcls ^ 3u) & cm) == 0u;
-                        const bool real2 = ok2 && vc;
+                        const uint32_t cc = vcls & cm;
+                        const uint32_t o = force ? force : (cc == 0u ? 1u : ((even && cc == cm) ? 2u : 0u));
+                        const double Dc = cu * (o == 2u ? d : c);
+                        const bool real2 = o == 2u && vc;
                         const double h2n = bd.h2_w - bd.tau;
-                        bool u1, u2;
-                        const uint32_t f1 = one ? o1_decide<true>(ok1, X, bd, u1) : o1_decide<false>(ok1, X, bd, u1);
-                        const uint32_t f2 = one ? o1_decide<true>(ok2 && !vc, Y, bd, u2)
-                                                : o1_decide<false>(ok2 && !vc, Y, bd, u2);
-                        const bool und = u1 || u2 || (real2 && bd.h2_w >= 0.0 && h2n < 0.0);
+                        bool u1;
+                        const bool live = o != 0u && !vc;
+                        const uint32_t f1 = one ? o1_decide<true>(live, Dc, bd, u1) : o1_decide<false>(live, Dc, bd, u1);
+                        const bool und = u1 || (real2 && bd.h2_w >= 0.0 && h2n < 0.0);
                         if (und) {
-                            defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 1, force ? force : 3u});
+                            defs.push_back({Tl.left, L.H - 1, vidx, k, 0, 1, o});
                             ++S.deferred_full;
                         } else {
-                            const uint32_t f = f1 | f2 << 2;
-                            cand[D.out] += __builtin_popcount(f) + ((real2 && h2n >= 0.0) ? 1u : 0u);
-                            root_test(Tl.left, L.H - 1, vidx, k, 0, f, X, Y, bd.lm);
+                            cand[D.out] += __builtin_popcount(f1) + ((real2 && h2n >= 0.0) ? 1u : 0u);
+                            root_test(Tl.left, L.H - 1, vidx, k, 0, f1, Dc, 0.0, bd.lm);
                         }
                     }
                     if (mode == 0) {
@@ -350,7 +348,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                     make_window_o1((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, bd.tau, 2.0 * D.l2_d * m2, w);
                     const double thv = theta_of_f(c, d);
                     for (uint32_t j = 0; j < allowed_count(cm, kmask); ++j)
-                        for (uint32_t rho = 0; rho < (even ? 2u : 1u); ++rho) {
+                        for (uint32_t rho = force ? force - 1 : 0u; rho < (force ? force : (even ? 2u : 1u)); ++rho) {
                             const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), cm, kmask, j), kmask);
                             double thw = thv + (rho ? HALF_PI : 0.0);
                             thw = thw >= PI_D ? thw - PI_D : thw;
