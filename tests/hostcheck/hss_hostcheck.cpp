@@ -124,6 +124,7 @@ typedef struct {
     uint64_t window_tiles;
     uint64_t window_items;   // leaves evaluated in window-mode tiles
     uint64_t class_mismatch; // leaves whose class-rule area decision differs from the residue test
+    uint64_t window_units;   // units of WIN_UNIT entries the window kernel runs for those leaves
 } hc_stats;
 
 static Tuning g_tune;
@@ -357,6 +358,7 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                                 uint32_t rs[4], re[4];
                                 window_o1_ranges(w, bt + g * nbg, nbg, thw, cj != 0, rs, re);
                                 const double de = cj ? -de0 : de0;
+                                for (int p = 0; p < 4; ++p) S.window_units += (re[p] - rs[p] + WIN_UNIT - 1) / WIN_UNIT;
                                 for (int p = 0; p < 4; ++p)
                                     for (uint32_t ui = rs[p]; ui < re[p]; ++ui) {
                                         ++S.window_items;

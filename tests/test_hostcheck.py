@@ -23,7 +23,8 @@ class HcHit(ctypes.Structure):
 class HcStats(ctypes.Structure):
     _fields_ = [("deferred_full", ctypes.c_uint64), ("deferred_root", ctypes.c_uint64),
                 ("raw_roots", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("window_tiles", ctypes.c_uint64),
-                ("window_items", ctypes.c_uint64), ("class_mismatch", ctypes.c_uint64)]
+                ("window_items", ctypes.c_uint64), ("class_mismatch", ctypes.c_uint64),
+                ("window_units", ctypes.c_uint64)]
 
 
 DEFAULT_TUNING = (0.5, 3, 8)  # threshold for weight classes >= 3 (H >= 64), tiles_per_wave (hss_plan.h Tuning)
