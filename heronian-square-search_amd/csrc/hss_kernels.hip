@@ -851,34 +851,40 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 // or outside the window) and are skipped; a leaf inside them with an
 // undecided band goes to the exact stage with its orientation only.
 
-// Partner entry of a chunk (phase 1 -> 2, 3).  Records are 16-byte aligned
-// and a multiple of 16 bytes long, so that every wide LDS access is aligned.
-struct alignas(16) WinPart {
-    double c, d;                   // v
+// Pair of a chunk lane's k (computed once per k, read by its entries in all
+// three phases).  Records are 16-byte aligned and a multiple of 16 bytes
+// long, so that every wide LDS access is aligned.
+struct alignas(16) WinK {
     double mid, half_w, h2_w, tau; // the pair's band (hss_math.h Band; l^4 + m^4 is recomputed from k)
     double wa, wla, wb;            // orientation-1 window arcs (Window1; two arcs have equal lengths)
+    uint32_t info;                 // cm | even << 2 | window arcs << 3
+    uint32_t kj;                   // k
+};
+static_assert(sizeof(WinK) == 64, "LDS budget of cell_win_kernel");
+
+// Partner entry of a chunk (phase 1 -> 2, 3).
+struct alignas(16) WinPart {
+    double c, d;                   // v
     double thv;                    // angle of v modulo pi
     uint32_t vidx;                 // v's cache entry
-    uint32_t kj;                   // k
-    uint32_t info;                 // class | cm << 2 | even << 4 | real v << 5 | window arcs << 6 | force << 8
-    uint32_t pad;
+    uint32_t info;                 // class | real v << 2 | force << 3 | k's chunk lane << 5
 };
-static_assert(sizeof(WinPart) == 96, "LDS budget of cell_win_kernel");
+static_assert(sizeof(WinPart) == 32, "LDS budget of cell_win_kernel");
 
 // Virtual partner of a batch (phase 2 -> 3): its ranges and partner lane.
-// 48 + 96 bytes per lane: four blocks of four waves fit a CU's 160 KB of LDS.
+// 64 + 32 + 48 bytes per lane: four blocks of four waves fit a CU's 160 KB of LDS.
 struct alignas(16) WinVp {
     uint32_t se[8];  // range p (0-3 leaves u w, 4-7 u conj(w)): start | end << 16 (left-set entry indices)
     uint32_t uc[4];  // units before ranges 2i (low half) and 2i + 1; range 0's (always 0) holds
-                     // the partner lane | rho << 6
+                     // the partner lane | rho << 6 | the lane of its k << 7
 };
 static_assert(sizeof(WinVp) == 48, "LDS budget of cell_win_kernel");
 
 // Phase 3: the flattened units of a batch of virtual partners.
 template <bool ONE>
-__device__ inline void win_units(const CellParams& P, const TileCtx& tc, const LeftFast* lf, const WinPart* part,
-                                 const WinVp* vps, uint32_t* mark, uint32_t uex, uint32_t units, uint32_t nunits,
-                                 uint32_t& cnt) {
+__device__ inline void win_units(const CellParams& P, const TileCtx& tc, const LeftFast* lf, const WinK* kws,
+                                 const WinPart* part, const WinVp* vps, uint32_t* mark, uint32_t uex, uint32_t units,
+                                 uint32_t nunits, uint32_t& cnt) {
     const uint32_t lane = __lane_id();
     uint32_t carry = 0;
     for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
@@ -905,13 +911,15 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         const uint32_t uend = liveu ? se >> 16 : 0u;
         const bool cj = pc >= 4;
         const WinPart& V = *(const WinPart*)((const char*)part + __umul24(pr & 63u, (uint32_t)sizeof(WinPart)));
+        const WinK& K = *(const WinK*)((const char*)kws + __umul24((pr >> 7) & 63u, (uint32_t)sizeof(WinK)));
         const bool rho = (pr >> 6) & 1u;
         // w = i^rho v; u conj(w) is u w with Im w negated: one sign per unit, none per leaf
         const double vc = V.c, vd = V.d;
         const double ce = rho ? -vd : vc, dw = rho ? vc : vd;
         const double de = cj ? -dw : dw;
-        const Band b{V.mid, V.half_w, V.h2_w, V.tau, lm_of(tc.q, V.kj, tc.l4_d)};
-        const uint32_t kinfo = V.kj | (rho ? DEF_O2 : DEF_O1) | (cj ? DEF_CONJ : 0u);
+        const uint32_t kj = K.kj;
+        const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(tc.q, kj, tc.l4_d)};
+        const uint32_t kinfo = kj | (rho ? DEF_O2 : DEF_O1) | (cj ? DEF_CONJ : 0u);
         const uint32_t vidx = V.vidx;
         const uint32_t nlive = uend > ui0 ? uend - ui0 : 0u;  // live entries of the unit
         // the unit's entries from one base: ui0 <= nsort, and the left entries
@@ -963,10 +971,12 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
 }
 
 __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
+    __shared__ WinK s_k[WAVES][64];
     __shared__ WinPart s_part[WAVES][64];
     __shared__ WinVp s_vp[WAVES][64];
     __shared__ uint32_t s_mark[WAVES][65];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WinK* kws = s_k[wave];
     WinPart* part = s_part[wave];
     WinVp* vps = s_vp[wave];
     uint32_t* mark = s_mark[wave];
@@ -986,51 +996,43 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        // an even pair with cm = 0 runs its entries twice, one orientation per run
-        const auto twice_of = [&](uint32_t k) {
-            const uint32_t info = pair_area_info(L, S, k);
-            return (info & 3u) == 0u && (info & 4u) != 0u;
+        // per k: the pair's band, area info and orientation-1 window; an even
+        // pair with cm = 0 runs its entries twice, one orientation per run
+        const auto on_chunk = [&](uint32_t k) {
+            const Band b = pair_band(L, S, k);
+            const uint32_t ainfo = pair_area_info(L, S, k);
+            const double md = (double)L.q * (double)k;  // exact: m < 2^26
+            const double m2 = md * md;                  // exact: m^2 < 2^52
+            Window1 w;
+            make_window_o1((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, b.tau, 2.0 * S.l2_d * m2, w);
+            __builtin_amdgcn_wave_barrier();
+            lds_at(kws, lane) = WinK{b.mid, b.half_w, b.h2_w, b.tau, w.a, w.la, w.b,
+                                     ainfo | (uint32_t)(w.n & 3) << 3, k};
+            __builtin_amdgcn_wave_barrier();
+            return ((ainfo & 3u) == 0u && (ainfo & 4u) != 0u) ? 2u : 1u;
         };
-        for_partner_chunks(P, T, L, S, mark, pairs, leaves, [&](uint32_t k) { return twice_of(k) ? 2u : 1u; },
-                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t, uint32_t oj) {
+        for_partner_chunks(P, T, L, S, mark, pairs, leaves, on_chunk,
+                           [&](bool act, uint32_t kj, uint32_t jj, uint32_t Hj, uint32_t j, uint32_t oj) {
             // ---- phase 1: lane = partner entry
-            const uint32_t ainfo = pair_area_info(L, S, kj);
-            const uint32_t cm = ainfo & 3u;
-            const bool even = (ainfo & 4u) != 0u;
+            const WinK& K = lds_at(kws, j);
+            const uint32_t kinf = K.info;
+            const uint32_t cm = kinf & 3u;
+            const bool even = (kinf & 4u) != 0u;
             const bool twice = cm == 0u && even;
             const uint32_t Hv = twice ? Hj >> 1 : Hj;
             const uint32_t force = twice ? (jj < Hv ? 1u : 2u) : 0u;
             Partner v;
             load_partner(P, act, kj, jj < Hv ? jj : jj - Hv, Hv, oj, v);
-            const Band b = pair_band(L, S, kj);
+            const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(L.q, kj, S.l4_d)};
             evald += (one ? central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt)
                           : central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt))
                          ? 1u
                          : 0u;
-            const double md = (double)L.q * (double)kj;  // exact: m < 2^26
-            const double m2 = md * md;                      // exact: m^2 < 2^52
-            Window1 w;
-            make_window_o1((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, b.tau, 2.0 * S.l2_d * m2, w);
             // virtual partners: class variants j times the orientations of this run
             const uint32_t nvp = act ? allowed_count(cm, kmask) << ((even && !force) ? 1 : 0) : 0u;
             __builtin_amdgcn_wave_barrier();
-            {
-                WinPart& R = part[lane];
-                R.c = v.c;
-                R.d = v.d;
-                R.mid = b.mid;
-                R.half_w = b.half_w;
-                R.h2_w = b.h2_w;
-                R.tau = b.tau;
-                R.wa = w.a;
-                R.wla = w.la;
-                R.wb = w.b;
-                R.thv = theta_of_f(v.c, v.d);
-                R.vidx = v.vidx;
-                R.kj = kj;
-                R.info = v.cls | cm << 2 | (even ? 16u : 0u) | (v.vc ? 32u : 0u) | (uint32_t)(w.n & 3) << 6 | force << 8;
-                R.pad = 0;
-            }
+            lds_at(part, lane) = WinPart{v.c, v.d, theta_of_f(v.c, v.d), v.vidx,
+                                         v.cls | (v.vc ? 4u : 0u) | force << 3 | j << 5};
             const uint32_t vincl = wave_incl_scan(nvp);
             const uint32_t vex = vincl - nvp;
             const uint32_t nvps = __builtin_amdgcn_readlane(vincl, 63);
@@ -1046,19 +1048,22 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const uint32_t q = livev ? vi - ovex : 0u;
                 const WinPart& V = *(const WinPart*)((const char*)part + __umul24(o, (uint32_t)sizeof(WinPart)));
                 const uint32_t info = V.info;
-                const uint32_t vcls = info & 3u, vcm = (info >> 2) & 3u;
-                const bool veven = (info >> 4) & 1u, vreal = (info >> 5) & 1u;
-                const uint32_t vforce = (info >> 8) & 3u;
+                const uint32_t kl = (info >> 5) & 63u;
+                const WinK& VK = *(const WinK*)((const char*)kws + __umul24(kl, (uint32_t)sizeof(WinK)));
+                const uint32_t vkinf = VK.info;
+                const uint32_t vcls = info & 3u, vcm = vkinf & 3u;
+                const bool veven = (vkinf >> 2) & 1u, vreal = (info >> 2) & 1u;
+                const uint32_t vforce = (info >> 3) & 3u;
                 const bool both = veven && !vforce;
-                const uint32_t rho = vforce ? vforce - 1u : (both ? (q & 1u) : 0u), j = both ? q >> 1 : q;
-                const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, j), kmask);
+                const uint32_t rho = vforce ? vforce - 1u : (both ? (q & 1u) : 0u), jv = both ? q >> 1 : q;
+                const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, jv), kmask);
                 const uint32_t* btg = bt + g * nbg;
                 Window1 vw;
-                vw.a = V.wa;
-                vw.la = V.wla;
-                vw.b = V.wb;
-                vw.lb = V.wla;
-                const uint32_t wn = (info >> 6) & 3u;
+                vw.a = VK.wa;
+                vw.la = VK.wla;
+                vw.b = VK.wb;
+                vw.lb = VK.wla;
+                const uint32_t wn = (vkinf >> 3) & 3u;
                 vw.n = wn == 3u ? -1 : (int)wn;
                 double thw = V.thv + (rho ? HALF_PI : 0.0);
                 thw = thw >= PI_D ? thw - PI_D : thw;
@@ -1073,7 +1078,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 for (int p = 0; p < 8; ++p) {
                     const uint32_t len = (p < 4 ? live0 : live1) ? re[p] - rs[p] : 0u;
                     R.se[p] = rs[p] | (rs[p] + len) << 16;
-                    ucs[p] = p ? units : (o | rho << 6);
+                    ucs[p] = p ? units : (o | rho << 6 | kl << 7);
                     items += len;
                     units += (len + WIN_UNIT - 1) / WIN_UNIT;
                 }
@@ -1086,9 +1091,9 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 __builtin_amdgcn_wave_barrier();
                 // ---- phase 3: lane = unit of WIN_UNIT consecutive entries of one range
                 if (one)
-                    win_units<true>(P, tc, lf, part, vps, mark, uex, units, nunits, cnt);
+                    win_units<true>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, cnt);
                 else
-                    win_units<false>(P, tc, lf, part, vps, mark, uex, units, nunits, cnt);
+                    win_units<false>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, cnt);
                 __builtin_amdgcn_wave_barrier();
             }
         });
