@@ -573,7 +573,7 @@ __device__ inline uint32_t pair_area_info(const LeftDesc& L, const PosDesc& S, u
 // pair's area info: one per lane of a partner chunk, shared by the k's
 // half-set entries.
 struct KCtx {
-    double mid, half_w, h2_w, tau;
+    double mid, half_w, h2_w, tau, lm;
     uint32_t info;  // pair_area_info | KCTX_TWICE
     uint32_t pad;
 };
@@ -642,6 +642,7 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
                                           uint32_t* mark, uint64_t& pairs, uint64_t& leaves, C&& on_chunk, F&& f) {
     const uint32_t lane = __lane_id();
     for (uint32_t par = 0; par < 2; ++par) {
+        if (S.l_even && par == 0u) continue;  // l and m both even: no pair
         const uint32_t kstart = T.k0 + ((T.k0 & 1u) != par ? 1u : 0u);
         for (uint32_t kb = kstart; kb <= T.k1; kb += 128) {
             const uint32_t k = kb + 2u * lane;
@@ -752,7 +753,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             const uint32_t info = pair_area_info(L, S, k);
             const bool twice = (info & 3u) == 0u && (info & 4u) != 0u;
             __builtin_amdgcn_wave_barrier();
-            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, info | (twice ? KCTX_TWICE : 0u), 0u};
+            lds_at(kctx, lane) = KCtx{c.mid, c.half_w, c.h2_w, c.tau, c.lm, info | (twice ? KCTX_TWICE : 0u), 0u};
             __builtin_amdgcn_wave_barrier();
             return twice ? 2u : 1u;
         };
@@ -766,7 +767,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             const uint32_t jv = jj < Hv ? jj : jj - Hv;
             Partner v;
             load_partner(P, act, kj, jv, Hv, oj, v);
-            const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(L.q, kj, S.l4_d)};
+            const Band b{K.mid, K.half_w, K.h2_w, K.tau, K.lm};
             const uint32_t cm = K.info & 3u;
             const bool even = (K.info & 4u) != 0u;
             // with cm != 0 a leaf passes the area tests in at most one
