@@ -41,7 +41,7 @@ inline int weight_class(uint32_t H) {
 // LIMIT 1e7 workload with another seed.
 struct Tuning {
     uint32_t tiles_per_wave = 32;  // tile size: est. work / (waves * tiles_per_wave)
-    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.45, 1.0, 0.5, 0.6, 1.0, 0.8, 0.5};
+    double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.8, 1.0, 0.5, 0.6, 1.0, 0.8, 0.5};
     double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf (tile order)
     double win_partner_cost = 4;  // per partner entry of a window tile, in plain-cell units
     double plain_partner_cost = 2.5;  // per partner entry of a plain tile, in plain-cell units
