@@ -902,12 +902,25 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
             uc[2 * i] = w & 0xFFFFu;
             uc[2 * i + 1] = w >> 16;
         }
-        uint32_t pc = 0;
-#pragma unroll
-        for (int p = 1; p < 8; ++p) pc += uc[p] <= wq ? 1u : 0u;
         const uint32_t pr = uc[0];
+        // the unit's range pc: the last p with units-before uc[p] <= wq (uc[0] counts as 0;
+        // uc never decreases), by a three-step search in registers; ucp = uc[pc]
+        uint32_t pc = 0, ucp = 0;
+        if (uc[4] <= wq) {
+            pc = 4;
+            ucp = uc[4];
+        }
+        const uint32_t x = pc ? uc[6] : uc[2];
+        if (x <= wq) {
+            pc += 2;
+            ucp = x;
+        }
+        const uint32_t y = pc == 0 ? uc[1] : (pc == 2 ? uc[3] : (pc == 4 ? uc[5] : uc[7]));
+        if (y <= wq) {
+            pc += 1;
+            ucp = y;
+        }
         const uint32_t se = Q.se[pc];
-        const uint32_t ucp = pc ? (Q.uc[pc >> 1] >> (16u * (pc & 1u))) & 0xFFFFu : 0u;
         const uint32_t ui0 = (se & 0xFFFFu) + (wq - ucp) * WIN_UNIT;
         const uint32_t uend = liveu ? se >> 16 : 0u;
         const bool cj = pc >= 4;
