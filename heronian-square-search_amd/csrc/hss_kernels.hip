@@ -7,8 +7,10 @@
 //   cre/cim/cres      the cache: for every i, the (w(i)+1)/2 Gaussian integers
 //                     fixed(i) * prod pi_p^r conj(pi_p)^(4e_p - r) whose digit
 //                     tuples are lexicographically <= the central tuple; FP64
-//                     coordinates (exact: |coord| <= i^2 < 2^53) and residues mod
-//                     180 (QA).  The last entry of each half-set is the real one.
+//                     coordinates (exact: |coord| <= i^2 < 2^53), the area class
+//                     (the class rule of hss_math.h) and residues mod 180 (QA;
+//                     the kernels read only the class).  The last entry of each
+//                     half-set is the real one.
 //   pval              per split prime p and merged exponent E <= 2*floor(log_p N):
 //                     pi^r conj(pi)^(4E-r), r = 0..4E, exact i128
 //
@@ -22,14 +24,15 @@
 // persistent grids pull tiles (left set x range of partner indices) from
 // device work queues.  In cell_kernel each wavefront maps one partner
 // half-set entry v per lane and walks the left set u uniformly (scalar loads),
-// evaluating the cells u*v and u*conj(v) with the FP64/residue fast path of
-// hss_math.h, which decides every candidate count exactly; cell_win_kernel
-// evaluates only the left entries inside each partner's angle window (left
-// sets sorted by angle).  Leaves with counted terms are compacted with
-// ballot/mbcnt into a per-wave LDS ring and root-tested a wave of terms at a
-// time; the few survivors, and the leaves left undecided by the FP64 band, go
-// to a global buffer that the exact kernel (k_exact) settles with 128/256-bit
-// integer arithmetic.
+// evaluating the cells u*v and u*conj(v) with the class rule and the FP64
+// band of hss_math.h, which decide every candidate count exactly;
+// cell_win_kernel evaluates only the left entries inside each partner's angle
+// window (left sets sorted by angle).  In cell_kernel rows with counted terms
+// are compacted with ballot/mbcnt into a per-wave LDS ring and root-tested a
+// wave of terms at a time (cell_win_kernel tests them inline); the few
+// survivors, and the leaves left undecided by the FP64 band, go to a global
+// buffer that the exact kernel (k_exact) settles with 128/256-bit integer
+// arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
