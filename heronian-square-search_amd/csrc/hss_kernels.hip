@@ -859,19 +859,21 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
 // three phases).  Records are 16-byte aligned and a multiple of 16 bytes
 // long, so that every wide LDS access is aligned.
 struct alignas(16) WinK {
-    double mid, half_w, h2_w, tau; // the pair's band (hss_math.h Band; l^4 + m^4 is recomputed from k)
-    double wa, wla, wb;            // orientation-1 window arcs (Window1; two arcs have equal lengths)
-    uint32_t info;                 // cm | even << 2 | window arcs << 3
-    uint32_t kj;                   // k
+    double mid, half_w, h2_w, tau, lm;  // the pair's band (hss_math.h Band)
+    uint32_t wa, wla, wb;               // orientation-1 window arcs in angle units (Window1F)
+    uint32_t info;                      // cm | even << 2 | window arcs << 3
+    uint32_t kj;                        // k
+    uint32_t pad;
 };
 static_assert(sizeof(WinK) == 64, "LDS budget of cell_win_kernel");
 
 // Partner entry of a chunk (phase 1 -> 2, 3).
 struct alignas(16) WinPart {
     double c, d;                   // v
-    double thv;                    // angle of v modulo pi
+    uint32_t thv;                  // angle of v modulo pi, angle units
     uint32_t vidx;                 // v's cache entry
     uint32_t info;                 // class | real v << 2 | force << 3 | k's chunk lane << 5
+    uint32_t pad;
 };
 static_assert(sizeof(WinPart) == 32, "LDS budget of cell_win_kernel");
 
@@ -935,7 +937,7 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         const double ce = rho ? -vd : vc, dw = rho ? vc : vd;
         const double de = cj ? -dw : dw;
         const uint32_t kj = K.kj;
-        const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(tc.q, kj, tc.l4_d)};
+        const Band b{K.mid, K.half_w, K.h2_w, K.tau, K.lm};
         const uint32_t kinfo = kj | (rho ? DEF_O2 : DEF_O1) | (cj ? DEF_CONJ : 0u);
         const uint32_t vidx = V.vidx;
         const uint32_t nlive = uend > ui0 ? uend - ui0 : 0u;  // live entries of the unit
@@ -1009,6 +1011,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const uint32_t nsort = L.H - 1;
         const uint32_t kmask = left_kmask(L);
         const uint32_t nbg = WIN_NB >> __popc(kmask);
+        const uint32_t bsh = 32u - (uint32_t)__builtin_ctz(nbg);  // bucket of an angle: its top bits
         const double cu = lf[nsort].re;  // the central entry is real
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
@@ -1022,9 +1025,11 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             const double m2 = md * md;                  // exact: m^2 < 2^52
             Window1 w;
             make_window_o1((S.l4_d + m2 * m2) - S.n4_d, S.l4_d, b.tau, 2.0 * S.l2_d * m2, w);
+            Window1F wf;
+            window_o1_fx(w, wf);
             __builtin_amdgcn_wave_barrier();
-            lds_at(kws, lane) = WinK{b.mid, b.half_w, b.h2_w, b.tau, w.a, w.la, w.b,
-                                     ainfo | (uint32_t)(w.n & 3) << 3, k};
+            lds_at(kws, lane) = WinK{b.mid, b.half_w, b.h2_w, b.tau, b.lm, wf.a, wf.la, wf.b,
+                                     ainfo | (uint32_t)(w.n & 3) << 3, k, 0u};
             __builtin_amdgcn_wave_barrier();
             return ((ainfo & 3u) == 0u && (ainfo & 4u) != 0u) ? 2u : 1u;
         };
@@ -1040,7 +1045,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             const uint32_t force = twice ? (jj < Hv ? 1u : 2u) : 0u;
             Partner v;
             load_partner(P, act, kj, jj < Hv ? jj : jj - Hv, Hv, oj, v);
-            const Band b{K.mid, K.half_w, K.h2_w, K.tau, lm_of(L.q, kj, S.l4_d)};
+            const Band b{K.mid, K.half_w, K.h2_w, K.tau, K.lm};
             evald += (one ? central_win<true>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt)
                           : central_win<false>(P, tc, act, cu, nsort, v, b, cm, even, force, cnt))
                          ? 1u
@@ -1048,8 +1053,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
             // virtual partners: class variants j times the orientations of this run
             const uint32_t nvp = act ? allowed_count(cm, kmask) << ((even && !force) ? 1 : 0) : 0u;
             __builtin_amdgcn_wave_barrier();
-            lds_at(part, lane) = WinPart{v.c, v.d, theta_of_f(v.c, v.d), v.vidx,
-                                         v.cls | (v.vc ? 4u : 0u) | force << 3 | j << 5};
+            lds_at(part, lane) = WinPart{v.c, v.d, theta_fx_of_f(v.c, v.d), v.vidx,
+                                         v.cls | (v.vc ? 4u : 0u) | force << 3 | j << 5, 0u};
             const uint32_t vincl = wave_incl_scan(nvp);
             const uint32_t vex = vincl - nvp;
             const uint32_t nvps = __builtin_amdgcn_readlane(vincl, 63);
@@ -1075,18 +1080,17 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const uint32_t rho = vforce ? vforce - 1u : (both ? (q & 1u) : 0u), jv = both ? q >> 1 : q;
                 const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, jv), kmask);
                 const uint32_t* btg = bt + g * nbg;
-                Window1 vw;
+                Window1F vw;
                 vw.a = VK.wa;
                 vw.la = VK.wla;
                 vw.b = VK.wb;
-                vw.lb = VK.wla;
                 const uint32_t wn = (vkinf >> 3) & 3u;
                 vw.n = wn == 3u ? -1 : (int)wn;
-                double thw = V.thv + (rho ? HALF_PI : 0.0);
-                thw = thw >= PI_D ? thw - PI_D : thw;
+                // w = i^rho v: its angle modulo pi wraps with the unsigned sum
+                const uint32_t thw = V.thv + (rho ? ANGLE_HALF_PI : 0u);
                 uint32_t rs[8], re[8];
-                const uint32_t n0 = window_o1_ranges(vw, btg, nbg, thw, false, rs, re);
-                const uint32_t n1 = window_o1_ranges(vw, btg, nbg, thw, true, rs + 4, re + 4);
+                const uint32_t n0 = window_o1_ranges(vw, btg, nbg, bsh, thw, false, rs, re);
+                const uint32_t n1 = window_o1_ranges(vw, btg, nbg, bsh, thw, true, rs + 4, re + 4);
                 const bool live0 = livev && n0 != 0, live1 = livev && !vreal && n1 != 0;
                 __builtin_amdgcn_wave_barrier();
                 WinVp& R = vps[lane];

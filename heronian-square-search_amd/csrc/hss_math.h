@@ -585,35 +585,74 @@ HSS_HD inline double window_o1_fraction(const Window1& w) {
     return f > 1.0 ? 1.0 : f;
 }
 
-// Entries of one group (bucket table slice btg[0..nbg], absolute entry
-// indices; btg[0] and btg[nbg] bound the group) whose theta lies in the cyclic
-// arc [a + shift, a + shift + len] (len < pi): one or two index ranges.
-HSS_HD inline void arc_ranges(const uint32_t* btg, uint32_t nbg, double a, double len, double shift, uint32_t& s0,
-                              uint32_t& e0, uint32_t& s1, uint32_t& e1) {
-    constexpr double INV_PI = 0.31830988618379067;
-    double x = a + shift;
-    x -= PI_D * floor(x * INV_PI);
-    x = x < 0.0 ? 0.0 : (x >= PI_D ? 0.0 : x);
-    const double y = x + len;
-    const bool wrap = y >= PI_D;
-    const uint32_t t = btg[theta_bucket(wrap ? y - PI_D : y, nbg) + 1];
-    s0 = btg[theta_bucket(x, nbg)];
+// Angles modulo pi in 32-bit fixed point for the range lookups of the window
+// kernel: 2^32 units per pi (one unit is 7.3e-10 rad), so reduction modulo pi
+// is the wraparound of unsigned arithmetic, pi/2 is 2^31, and the bucket of an
+// angle among nb = 2^b buckets is its top b bits (the bucket theta_bucket
+// gives the same angle, up to the conversion below).  The conversions floor
+// (FP64 products: within one unit, plus one for the partner's angle in the
+// shift), and the arcs are widened outward by two units at the start and four
+// in length, so a fixed-point arc encloses the FP64 one.  These units add
+// < 3e-9 rad to the window's error budget, inside the 2e-6 widening.
+constexpr double ANGLE_UNITS_PER_RAD = 4294967296.0 / PI_D;  // 2^32 / pi
+constexpr uint32_t ANGLE_HALF_PI = 0x80000000u;
+
+// floor(t * 2^32 / pi) for t in (-pi, 2 pi) reduced modulo pi; an angle that
+// rounds to pi is 0 (the same point of the circle).
+HSS_HD inline uint32_t angle_fx(double t) {
+    t = t < 0.0 ? t + PI_D : (t >= PI_D ? t - PI_D : t);
+    const double u = t * ANGLE_UNITS_PER_RAD;
+    return u < 4294967296.0 ? (uint32_t)u : 0u;
+}
+
+// theta of (x, y) != (0, 0) modulo pi from FP32 coordinates (theta_of_f), in
+// angle units.
+HSS_HD inline uint32_t theta_fx_of_f(double x, double y) {
+    const bool odd = (x <= 0 && y > 0) || (x >= 0 && y < 0);
+    return angle_fx(psi_of_f(x, y)) + (odd ? ANGLE_HALF_PI : 0u);
+}
+
+// Window1 in angle units: arcs [a, a + la] and [b, b + la] (the two arcs of a
+// two-arc window have equal lengths), n as in Window1.
+struct Window1F {
+    uint32_t a, la, b;
+    int n;
+};
+
+HSS_HD inline void window_o1_fx(const Window1& w, Window1F& f) {
+    f.a = angle_fx(w.a) - 2u;
+    f.b = angle_fx(w.b) - 2u;
+    // la < pi - WIN_GAP whenever it is used (n >= 1); the whole circle ignores it
+    const double la = w.la * ANGLE_UNITS_PER_RAD;
+    f.la = la < 4294967000.0 ? (uint32_t)la + 4u : 4294967000u;
+    f.n = w.n;
+}
+
+// arc_ranges in angle units: entries of one group (bucket slice btg[0..nbg],
+// nbg = 2^(32 - sh)) whose theta lies in the cyclic arc [a + shift, a + shift
+// + len]: the arc wraps past pi when the unsigned sum does.
+HSS_HD inline void arc_ranges_fx(const uint32_t* btg, uint32_t nbg, uint32_t sh, uint32_t a, uint32_t len,
+                                 uint32_t shift, uint32_t& s0, uint32_t& e0, uint32_t& s1, uint32_t& e1) {
+    const uint32_t x = a + shift, y = x + len;
+    const bool wrap = y < x;
+    const uint32_t t = btg[(y >> sh) + 1u];
+    s0 = btg[x >> sh];
     e0 = wrap ? btg[nbg] : t;
     s1 = btg[0];
     e1 = wrap ? t : btg[0];
 }
 
 // Up to four disjoint index ranges of one group for the leaves u * w, w the
-// partner (rotated by i for orientation 2) of angle theta_w, or u * conj(w)
-// (conj): theta_u in W - theta_w, or W + theta_w.  The arcs and their wrapped
-// pieces are at least WIN_GAP (two buckets) apart, so the bucket-rounded
-// ranges are disjoint.  Returns the entries covered.
-HSS_HD inline uint32_t window_o1_ranges(const Window1& w, const uint32_t* btg, uint32_t nbg, double theta_w, bool conj,
-                                        uint32_t* s, uint32_t* e) {
-    const double shift = conj ? theta_w : -theta_w;
+// partner (rotated by i for orientation 2) of angle theta_w (angle units), or
+// u * conj(w) (conj): theta_u in W - theta_w, or W + theta_w.  The arcs and
+// their wrapped pieces are at least WIN_GAP (two buckets) apart, so the
+// bucket-rounded ranges are disjoint.  Returns the entries covered.
+HSS_HD inline uint32_t window_o1_ranges(const Window1F& w, const uint32_t* btg, uint32_t nbg, uint32_t sh,
+                                        uint32_t theta_w, bool conj, uint32_t* s, uint32_t* e) {
+    const uint32_t shift = conj ? theta_w : 0u - theta_w;
     uint32_t s0, e0, s1, e1, s2, e2, s3, e3;
-    arc_ranges(btg, nbg, w.a, w.la, shift, s0, e0, s1, e1);
-    arc_ranges(btg, nbg, w.b, w.lb, shift, s2, e2, s3, e3);
+    arc_ranges_fx(btg, nbg, sh, w.a, w.la, shift, s0, e0, s1, e1);
+    arc_ranges_fx(btg, nbg, sh, w.b, w.la, shift, s2, e2, s3, e3);
     const bool all = w.n < 0, two = w.n == 2;
     s[0] = all ? btg[0] : s0;
     e[0] = all ? btg[nbg] : e0;

@@ -347,16 +347,18 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                     // cell_win_kernel: passes over (class variant j, orientation rho)
                     Window1 w;
                     make_window_o1((D.l4_d + m2 * m2) - D.n4_d, D.l4_d, bd.tau, 2.0 * D.l2_d * m2, w);
-                    const double thv = theta_of_f(c, d);
+                    Window1F wf;
+                    window_o1_fx(w, wf);
+                    const uint32_t thv = theta_fx_of_f(c, d);
+                    const uint32_t bsh = 32u - (uint32_t)__builtin_ctz(nbg);
                     for (uint32_t j = 0; j < allowed_count(cm, kmask); ++j)
                         for (uint32_t rho = force ? force - 1 : 0u; rho < (force ? force : (even ? 2u : 1u)); ++rho) {
                             const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), cm, kmask, j), kmask);
-                            double thw = thv + (rho ? HALF_PI : 0.0);
-                            thw = thw >= PI_D ? thw - PI_D : thw;
+                            const uint32_t thw = thv + (rho ? ANGLE_HALF_PI : 0u);
                             const double ce = rho ? -d : c, de0 = rho ? c : d;
                             for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
                                 uint32_t rs[4], re[4];
-                                window_o1_ranges(w, bt + g * nbg, nbg, thw, cj != 0, rs, re);
+                                window_o1_ranges(wf, bt + g * nbg, nbg, bsh, thw, cj != 0, rs, re);
                                 const double de = cj ? -de0 : de0;
                                 for (int p = 0; p < 4; ++p) S.window_units += (re[p] - rs[p] + WIN_UNIT - 1) / WIN_UNIT;
                                 for (int p = 0; p < 4; ++p)
