@@ -1079,7 +1079,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 const bool both = veven && !vforce;
                 const uint32_t rho = vforce ? vforce - 1u : (both ? (q & 1u) : 0u), jv = both ? q >> 1 : q;
                 const uint32_t g = area_group(allowed_class(vcls ^ (rho ? 3u : 0u), vcm, kmask, jv), kmask);
-                const uint32_t* btg = bt + g * nbg;
+                const uint32_t g0 = g * nbg;
+                const BucketGroup G{bt, g0, bsh, bucket_at(bt, g0), bucket_at(bt, g0 + nbg)};
                 Window1F vw;
                 vw.a = VK.wa;
                 vw.la = VK.wla;
@@ -1089,8 +1090,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 // w = i^rho v: its angle modulo pi wraps with the unsigned sum
                 const uint32_t thw = V.thv + (rho ? ANGLE_HALF_PI : 0u);
                 uint32_t rs[8], re[8];
-                const uint32_t n0 = window_o1_ranges(vw, btg, nbg, bsh, thw, false, rs, re);
-                const uint32_t n1 = window_o1_ranges(vw, btg, nbg, bsh, thw, true, rs + 4, re + 4);
+                const uint32_t n0 = window_o1_ranges(vw, G, thw, false, rs, re);
+                const uint32_t n1 = window_o1_ranges(vw, G, thw, true, rs + 4, re + 4);
                 const bool live0 = livev && n0 != 0, live1 = livev && !vreal && n1 != 0;
                 __builtin_amdgcn_wave_barrier();
                 WinVp& R = vps[lane];

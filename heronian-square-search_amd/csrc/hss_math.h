@@ -628,18 +628,33 @@ HSS_HD inline void window_o1_fx(const Window1& w, Window1F& f) {
     f.n = w.n;
 }
 
-// arc_ranges in angle units: entries of one group (bucket slice btg[0..nbg],
-// nbg = 2^(32 - sh)) whose theta lies in the cyclic arc [a + shift, a + shift
-// + len]: the arc wraps past pi when the unsigned sum does.
-HSS_HD inline void arc_ranges_fx(const uint32_t* btg, uint32_t nbg, uint32_t sh, uint32_t a, uint32_t len,
-                                 uint32_t shift, uint32_t& s0, uint32_t& e0, uint32_t& s1, uint32_t& e1) {
+// Element i of a bucket table through a 32-bit byte offset from its base (on
+// the device a scalar base plus a per-lane offset, no 64-bit address
+// arithmetic per lane).
+HSS_HD inline uint32_t bucket_at(const uint32_t* bt, uint32_t i) {
+    return *(const uint32_t*)((const char*)bt + (i << 2));
+}
+
+// One group of a bucket table: its slice starts at entry g0 of bt and has nbg
+// = 2^(32 - sh) buckets; lo = bt[g0] and hi = bt[g0 + nbg] bound the group's
+// entries.
+struct BucketGroup {
+    const uint32_t* bt;
+    uint32_t g0, sh, lo, hi;
+};
+
+// arc_ranges in angle units: entries of one group whose theta lies in the
+// cyclic arc [a + shift, a + shift + len]: the arc wraps past pi when the
+// unsigned sum does.  One or two index ranges.
+HSS_HD inline void arc_ranges_fx(const BucketGroup& G, uint32_t a, uint32_t len, uint32_t shift, uint32_t& s0,
+                                 uint32_t& e0, uint32_t& s1, uint32_t& e1) {
     const uint32_t x = a + shift, y = x + len;
     const bool wrap = y < x;
-    const uint32_t t = btg[(y >> sh) + 1u];
-    s0 = btg[x >> sh];
-    e0 = wrap ? btg[nbg] : t;
-    s1 = btg[0];
-    e1 = wrap ? t : btg[0];
+    const uint32_t t = bucket_at(G.bt, G.g0 + (y >> G.sh) + 1u);
+    s0 = bucket_at(G.bt, G.g0 + (x >> G.sh));
+    e0 = wrap ? G.hi : t;
+    s1 = G.lo;
+    e1 = wrap ? t : G.lo;
 }
 
 // Up to four disjoint index ranges of one group for the leaves u * w, w the
@@ -647,17 +662,17 @@ HSS_HD inline void arc_ranges_fx(const uint32_t* btg, uint32_t nbg, uint32_t sh,
 // u * conj(w) (conj): theta_u in W - theta_w, or W + theta_w.  The arcs and
 // their wrapped pieces are at least WIN_GAP (two buckets) apart, so the
 // bucket-rounded ranges are disjoint.  Returns the entries covered.
-HSS_HD inline uint32_t window_o1_ranges(const Window1F& w, const uint32_t* btg, uint32_t nbg, uint32_t sh,
-                                        uint32_t theta_w, bool conj, uint32_t* s, uint32_t* e) {
+HSS_HD inline uint32_t window_o1_ranges(const Window1F& w, const BucketGroup& G, uint32_t theta_w, bool conj,
+                                        uint32_t* s, uint32_t* e) {
     const uint32_t shift = conj ? theta_w : 0u - theta_w;
     uint32_t s0, e0, s1, e1, s2, e2, s3, e3;
-    arc_ranges_fx(btg, nbg, sh, w.a, w.la, shift, s0, e0, s1, e1);
-    arc_ranges_fx(btg, nbg, sh, w.b, w.la, shift, s2, e2, s3, e3);
+    arc_ranges_fx(G, w.a, w.la, shift, s0, e0, s1, e1);
+    arc_ranges_fx(G, w.b, w.la, shift, s2, e2, s3, e3);
     const bool all = w.n < 0, two = w.n == 2;
-    s[0] = all ? btg[0] : s0;
-    e[0] = all ? btg[nbg] : e0;
-    s[1] = all ? btg[0] : s1;
-    e[1] = all ? btg[0] : e1;
+    s[0] = all ? G.lo : s0;
+    e[0] = all ? G.hi : e0;
+    s[1] = all ? G.lo : s1;
+    e[1] = all ? G.lo : e1;
     s[2] = two ? s2 : 0u;
     e[2] = two ? e2 : 0u;
     s[3] = two ? s3 : 0u;

@@ -358,7 +358,8 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
                             const double ce = rho ? -d : c, de0 = rho ? c : d;
                             for (uint32_t cj = 0; cj < (vc ? 1u : 2u); ++cj) {
                                 uint32_t rs[4], re[4];
-                                window_o1_ranges(wf, bt + g * nbg, nbg, bsh, thw, cj != 0, rs, re);
+                                const BucketGroup G{bt, g * nbg, bsh, bt[g * nbg], bt[g * nbg + nbg]};
+                                window_o1_ranges(wf, G, thw, cj != 0, rs, re);
                                 const double de = cj ? -de0 : de0;
                                 for (int p = 0; p < 4; ++p) S.window_units += (re[p] - rs[p] + WIN_UNIT - 1) / WIN_UNIT;
                                 for (int p = 0; p < 4; ++p)
