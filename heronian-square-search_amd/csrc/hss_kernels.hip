@@ -486,6 +486,7 @@ struct TileCtx {
     uint32_t left;
     uint32_t q;      // m = q * k
     double l4_d;     // FP64 l^4
+    double tol2;     // root-test tolerance of counted second-sign terms (root_tol2)
 };
 
 
@@ -628,7 +629,7 @@ __device__ inline bool central_win(const CellParams& P, const TileCtx& tc, bool 
     bool pass = false;
     if (__builtin_amdgcn_ballot_w64(f & 1u)) pass = (f & 1u) && root_near_integer(__builtin_fma(-2.0, a, b.lm));
     if (!ONE && __builtin_amdgcn_ballot_w64(f & 2u))
-        pass |= (f & 2u) && root_near_integer(__builtin_fma(2.0, a, b.lm));
+        pass |= (f & 2u) && root_near_integer(__builtin_fma(2.0, a, b.lm), tc.tol2);
     if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
     return (ok && !v.vc) || real2;  // the leaf had an orientation to test
 }
@@ -682,8 +683,8 @@ __device__ inline void for_partner_chunks(const CellParams& P, const TileDesc& T
 
 // Tile bookkeeping shared by both cell kernels: per-position counters and the
 // kernel's own totals (kstat[0]: evaluated leaf orientations, kstat[2]:
-// candidates).  cand = the lanes' direct counts cnt plus the queue's tested
-// terms.
+// candidates).  cand = the lanes' direct counts cnt plus a count kept per
+// wave (the root queue's tested terms, or the window units' counted terms).
 __device__ inline void tile_done(const CellParams& P, const PosDesc& S, uint32_t cnt, uint64_t tested,
                                  uint64_t leaves, uint64_t pairs, uint64_t evald, unsigned long long* kstat) {
     const uint32_t lane = __lane_id();
@@ -724,7 +725,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const TileDesc T = tiles[t];
         const LeftDesc L = lefts[T.left];
         const PosDesc& S = pos[L.pos];
-        const TileCtx tc{T.left, L.q, S.l4_d};
+        const TileCtx tc{T.left, L.q, S.l4_d, root_tol2(S.n4_d)};
         const LeftFast* lf = lfast + L.off;
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
@@ -890,7 +891,7 @@ static_assert(sizeof(WinVp) == 48, "LDS budget of cell_win_kernel");
 template <bool ONE>
 __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const LeftFast* lf, const WinK* kws,
                                  const WinPart* part, const WinVp* vps, uint32_t* mark, uint32_t uex, uint32_t units,
-                                 uint32_t nunits, uint32_t& cnt) {
+                                 uint32_t nunits, uint64_t& wcnt) {
     const uint32_t lane = __lane_id();
     uint32_t carry = 0;
     for (uint32_t w0 = 0; w0 < nunits; w0 += 64) {
@@ -908,23 +909,13 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
             uc[2 * i + 1] = w >> 16;
         }
         const uint32_t pr = uc[0];
-        // the unit's range pc: the last p with units-before uc[p] <= wq (uc[0] counts as 0;
-        // uc never decreases), by a three-step search in registers; ucp = uc[pc]
-        uint32_t pc = 0, ucp = 0;
-        if (uc[4] <= wq) {
-            pc = 4;
-            ucp = uc[4];
-        }
-        const uint32_t x = pc ? uc[6] : uc[2];
-        if (x <= wq) {
-            pc += 2;
-            ucp = x;
-        }
-        const uint32_t y = pc == 0 ? uc[1] : (pc == 2 ? uc[3] : (pc == 4 ? uc[5] : uc[7]));
-        if (y <= wq) {
-            pc += 1;
-            ucp = y;
-        }
+        // the unit's range pc: the number of ranges p >= 1 whose units-before
+        // uc[p] <= wq (uc never decreases), compares without branches; ucp =
+        // uc[pc] read back by its halfword address (range 0 starts at unit 0)
+        uint32_t pc = 0;
+#pragma unroll
+        for (int p = 1; p < 8; ++p) pc += uc[p] <= wq ? 1u : 0u;
+        const uint32_t ucp = pc ? (uint32_t)*(const uint16_t*)((const char*)Q.uc + 2u * pc) : 0u;
         const uint32_t se = Q.se[pc];
         const uint32_t ui0 = (se & 0xFFFFu) + (wq - ucp) * WIN_UNIT;
         const uint32_t uend = liveu ? se >> 16 : 0u;
@@ -963,12 +954,15 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
             for (uint32_t s = 0; s < WIN_UNIT; ++s)
                 push_global(P, U[s], DefItem{tc.left, ui0 + s, vidx, kinfo | DEF_FULL});
         }
+        // the candidates: counted terms summed over the wave (scalar popcounts of the lane masks)
         bool pass[WIN_UNIT];
         bool pany = false;
+        const double tol1 = root_tol1(b.lm);
 #pragma unroll
         for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-            cnt += (uint32_t)__popc(F[s]);
-            pass[s] = (F[s] & 1u) && root_near_integer(__builtin_fma(-2.0, AX[s], b.lm));
+            wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 1u) != 0u));
+            if constexpr (!ONE) wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 2u) != 0u));
+            pass[s] = (F[s] & 1u) && root_near_integer(__builtin_fma(-2.0, AX[s], b.lm), tol1);
         }
         if constexpr (!ONE) {
             bool any2 = false;
@@ -977,7 +971,7 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
             if (__builtin_amdgcn_ballot_w64(any2)) {
 #pragma unroll
                 for (uint32_t s = 0; s < WIN_UNIT; ++s)
-                    pass[s] |= (F[s] & 2u) && root_near_integer(__builtin_fma(2.0, AX[s], b.lm));
+                    pass[s] |= (F[s] & 2u) && root_near_integer(__builtin_fma(2.0, AX[s], b.lm), tc.tol2);
             }
         }
 #pragma unroll
@@ -1005,7 +999,7 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const TileDesc T = tiles[t];
         const LeftDesc L = lefts[T.left];
         const PosDesc& S = pos[L.pos];
-        const TileCtx tc{T.left, L.q, S.l4_d};
+        const TileCtx tc{T.left, L.q, S.l4_d, root_tol2(S.n4_d)};
         const LeftFast* lf = lfast + L.off;
         const uint32_t* bt = P.btab + (size_t)L.bt * (WIN_NB + 1);
         const uint32_t nsort = L.H - 1;
@@ -1014,7 +1008,8 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
         const uint32_t bsh = 32u - (uint32_t)__builtin_ctz(nbg);  // bucket of an angle: its top bits
         const double cu = lf[nsort].re;  // the central entry is real
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
-        uint32_t cnt = 0;
+        uint32_t cnt = 0;     // per lane: candidates of the central entry's leaves
+        uint64_t wcnt = 0;    // per wave: candidates of the units
         uint64_t leaves = 0, pairs = 0, evald = 0;
         // per k: the pair's band, area info and orientation-1 window; an even
         // pair with cm = 0 runs its entries twice, one orientation per run
@@ -1113,13 +1108,13 @@ __global__ __launch_bounds__(WG) void cell_win_kernel(HSS_CELL_ARGS) {
                 __builtin_amdgcn_wave_barrier();
                 // ---- phase 3: lane = unit of WIN_UNIT consecutive entries of one range
                 if (one)
-                    win_units<true>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, cnt);
+                    win_units<true>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, wcnt);
                 else
-                    win_units<false>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, cnt);
+                    win_units<false>(P, tc, lf, kws, part, vps, mark, uex, units, nunits, wcnt);
                 __builtin_amdgcn_wave_barrier();
             }
         });
-        tile_done(P, S, cnt, 0, leaves, pairs, evald, P.evaluated);
+        tile_done(P, S, cnt, wcnt, leaves, pairs, evald, P.evaluated);
     }
 }
 

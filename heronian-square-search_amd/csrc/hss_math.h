@@ -178,18 +178,23 @@ HSS_HD inline double root4_refine(double zz, double t, double r) {
     return __builtin_fma(-d, r * w, r);
 }
 
-// Is zz within 2^-47 zz of the fourth power of n = rint(r)?  n^2 is exact
-// (n <= 2^26) and the fma rounds once.  A true fourth power Z = n^4 estimated
-// with |zz - Z| < 2^-49.3 Z passes; a random zz passes with probability
-// ~2^-48 n.
-HSS_HD inline bool root_candidate_ok(double zz, double r) {
+// Is zz within tol of the fourth power of n = rint(r)?  n^2 is exact (n <=
+// 2^26) and the fma rounds once.  With tol = 2^-47 Z for any Z >= zz, a true
+// fourth power n^4 estimated with |zz - n^4| < 2^-49.3 n^4 passes; a random zz
+// passes with probability ~2^-48 n Z / zz.  The kernels take Z per pair
+// (l^4 + m^4 bounds the first-sign terms zz = l^4 + m^4 - 2|D|, 2 N^4 the
+// counted second-sign terms), which saves scaling zz per term.
+HSS_HD inline bool root_candidate_ok(double zz, double r, double tol) {
     const double n = __builtin_rint(r);
     const double n2 = n * n;
     const double e = __builtin_fma(n2, n2, -zz);
-    return __builtin_fabs(e) <= zz * 0x1p-47;
+    return __builtin_fabs(e) <= tol;
 }
 
-HSS_HD inline bool root_near_integer(double zz) {
+HSS_HD inline bool root_candidate_ok(double zz, double r) { return root_candidate_ok(zz, r, zz * 0x1p-47); }
+
+// tol as in root_candidate_ok (device); the host form ignores it
+HSS_HD inline bool root_near_integer(double zz, double tol) {
 #if defined(__HIP_DEVICE_COMPILE__)
     // zz^(1/4) from two hardware reciprocal square roots (t = zz^-1/2, then
     // t^-1/2), each within 2^-22 relative on a pessimistic reading of their
@@ -198,13 +203,21 @@ HSS_HD inline bool root_near_integer(double zz) {
     // for n <= 2^26, and rint(r) is the only candidate n.  (Even with the
     // hardware bound 2^6 times worse the step stays within 0.2 absolute.)
     const double t = __builtin_amdgcn_rsq(zz);
-    return root_candidate_ok(zz, root4_refine(zz, t, __builtin_amdgcn_rsq(t)));
+    return root_candidate_ok(zz, root4_refine(zz, t, __builtin_amdgcn_rsq(t)), tol);
 #else
+    (void)tol;
     double r = __builtin_sqrt(__builtin_sqrt(zz));
     double f = r - __builtin_rint(r);
     return (f < 0 ? -f : f) <= ROOT_EPS;
 #endif
 }
+
+HSS_HD inline bool root_near_integer(double zz) { return root_near_integer(zz, zz * 0x1p-47); }
+
+// Tolerances of the root test for a pair: first-sign terms (zz <= l^4 + m^4)
+// and counted second-sign terms (zz <= N^4 up to the estimate's error).
+HSS_HD inline double root_tol1(double lm) { return lm * 0x1p-47; }
+HSS_HD inline double root_tol2(double n4_d) { return n4_d * 0x1p-46; }
 
 // ----------------------------------------------------------- range windows
 //
