@@ -75,6 +75,8 @@ void Sched::set_costs(uint64_t id, std::vector<double> cost) {
     auto it = active_.find(id);
     if (it == active_.end()) return;
     cost.resize(it->second.npos, 1.0);
+    it->second.left = 0;
+    for (uint32_t i = it->second.cursor; i < it->second.stop; ++i) it->second.left += cost[i];
     it->second.cost = std::move(cost);
     it->second.costed = true;
 }
@@ -95,6 +97,7 @@ int Sched::cancel(uint64_t id) {
     if (it == active_.end()) return 0;
     Active& a = it->second;
     a.stop = a.cursor;
+    a.left = 0;
     return a.done == a.stop ? 2 : 3;
 }
 
@@ -104,7 +107,10 @@ std::vector<Segment> Sched::next_launch() {
         if (kv.second.costed && kv.second.cursor < kv.second.stop) live.emplace_back(kv.first, &kv.second);
     std::vector<Segment> segs;
     if (live.empty()) return segs;
-    const double budget = cfg_.launch_work;
+    // a request that can finish within a quarter of the budget: a half-size launch
+    bool short_left = false;
+    for (const auto& [id, a] : live) short_left = short_left || a->left <= 0.25 * cfg_.launch_work;
+    const double budget = short_left ? 0.5 * cfg_.launch_work : cfg_.launch_work;
     const double share = budget / (double)live.size();
     double total = 0;
     uint32_t positions = 0;
@@ -118,18 +124,25 @@ std::vector<Segment> Sched::next_launch() {
             const double w = a.cost[a.cursor];
             if (a.cursor > start[i] && acc + w > share) break;
             acc += w;
+            a.left -= w;
             ++a.cursor;
             ++positions;
         }
         total += acc;
     }
-    // pass 2: the rest of the budget in admission order
-    for (size_t i = 0; i < live.size() && positions < cfg_.max_launch_positions; ++i) {
+    // pass 2: the rest of the budget, least work left first (admission order on ties)
+    std::vector<size_t> order(live.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return live[x].second->left < live[y].second->left; });
+    for (size_t i : order) {
+        if (positions >= cfg_.max_launch_positions) break;
         Active& a = *live[i].second;
         while (a.cursor < a.stop && positions < cfg_.max_launch_positions) {
             const double w = a.cost[a.cursor];
             if (total + w > budget) break;
             total += w;
+            a.left -= w;
             ++a.cursor;
             ++positions;
         }

@@ -7,11 +7,15 @@
 // has a contiguous free range for all its positions.  Each launch takes work
 // from every admitted request that still has positions left: first an equal
 // share of the launch's work budget per request (at least one position each),
-// then the rest of the budget in admission order.  Requests therefore join
-// the launch stream as soon as they are admitted and short requests are not
-// held behind long ones.  Positions of one request are handed out in order;
-// launches can finish out of order (a launch that is run again), so the
-// finished prefix [first, resume) is tracked separately from the count.
+// then the rest of the budget to the requests with the least work left.
+// Requests therefore join the launch stream as soon as they are admitted and
+// short requests are not held behind long ones.  While a request could finish
+// within a quarter of the budget, launches take half the budget, so that it
+// finishes after a short launch (launches of long requests alone stay long:
+// fewer, larger launches for throughput).  Positions of one request are
+// handed out in order; launches can finish out of order (a launch that is run
+// again), so the finished prefix [first, resume) is tracked separately from
+// the count.
 //
 // No counterpart in the reference, which runs one interval per process
 // (scripts/gpu_queue.py:65-98 starts one gpu_search_v42 per GPU).
@@ -78,6 +82,7 @@ private:
         uint32_t cursor = 0;  // positions handed to launches
         uint32_t done = 0;    // positions in finished launches
         bool costed = false;
+        double left = 0;  // work of the positions not handed out yet
         std::vector<double> cost;
         std::map<uint32_t, uint32_t> fin;  // finished ranges, merged: begin -> end
     };

@@ -12,6 +12,8 @@
 //     grid's work queue;
 //   * two launch buffer sets: while one launch runs, the next is planned on
 //     the host and queued behind it, so the device never waits for the host;
+//     it is planned late, shortly before the running launch is predicted to
+//     end (plan_delay), so that requests arriving meanwhile join it;
 //   * results: counters accumulate in the pool; when a request's last
 //     positions finish, its range is read back on a second stream (without
 //     waiting for the launch that is running) and the request completes.
@@ -61,6 +63,21 @@ struct Launch {
     std::vector<Segment> segs;
     std::vector<uint64_t> owner;  // plan position -> request id
     int attempts = 0;
+    double work = 0;      // planner work estimate of the launch
+    double t_start = -1;  // server clock when the device reached it (-1: not seen yet)
+};
+
+// Launch timing the worker learns as it goes: device milliseconds per unit of
+// planner work, and host milliseconds to plan and enqueue a launch.
+struct LaunchClock {
+    double ms_per_work = 0;  // 0 until a launch has finished
+    double plan_ms = 2.0;
+    void finished(double work, double device_ms) {
+        if (work <= 0 || device_ms <= 0) return;
+        const double r = device_ms / work;
+        ms_per_work = ms_per_work > 0 ? 0.7 * ms_per_work + 0.3 * r : r;
+    }
+    void planned(double host_ms) { plan_ms = 0.7 * plan_ms + 0.3 * host_ms; }
 };
 
 }  // namespace
@@ -173,6 +190,29 @@ void finalize_locked(hss_server* S, uint64_t id) {
     S->cv_work.notify_all();
 }
 
+// Late planning.  With one launch on the device and the other buffer set
+// free, the next launch is planned only shortly before the running one is
+// predicted to end (its start, seen through its first event, plus the learned
+// milliseconds per unit of work, minus a lead that covers planning and
+// enqueueing), so requests that arrive meanwhile still join it: a request
+// then waits for about the rest of the running launch and its own, not for a
+// launch already queued.  Returns the milliseconds to wait before planning
+// (<= 0: plan now).  A launch that has finished, or a clock that has no
+// estimate yet, plans at once, so the device never waits for the host longer
+// than one planning.
+double plan_delay(hss_server* S, std::deque<Launch>& inflight, const LaunchClock& clk) {
+    if (clk.ms_per_work <= 0 || inflight.size() != 1) return 0;
+    Launch& L = inflight.front();
+    const LaunchBuffers& B = S->bufs[L.buf];
+    if (hipEventQuery(B.ev[5]) == hipSuccess) return 0;
+    if (L.t_start < 0) {
+        if (hipEventQuery(B.ev[0]) != hipSuccess) return 0.25;  // queued behind copies: look again shortly
+        L.t_start = S->now();
+    }
+    const double lead = 2.0 * clk.plan_ms + 1.0;
+    return L.t_start + clk.ms_per_work * L.work - lead - S->now();
+}
+
 void worker_main(hss_server* S) {
     hss_ctx* c = S->ctx;
     if (hipSetDevice(c->device) != hipSuccess) {
@@ -180,6 +220,7 @@ void worker_main(hss_server* S) {
         return;
     }
     std::deque<Launch> inflight;
+    LaunchClock clk;
     unsigned free_mask = 3;
     for (;;) {
         std::vector<uint64_t> admitted;
@@ -214,8 +255,16 @@ void worker_main(hss_server* S) {
             }
         }
         if (free_mask) {
-            std::lock_guard<std::mutex> lk(S->mu);
+            std::unique_lock<std::mutex> lk(S->mu);
             for (size_t a = 0; a < admitted.size(); ++a) S->sched.set_costs(admitted[a], std::move(costs[a]));
+            if (S->sched.has_work() && !S->stop) {
+                const double wait_ms = plan_delay(S, inflight, clk);
+                if (wait_ms > 0) {
+                    // wake for submissions; look at the running launch at least every millisecond
+                    S->cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(std::min(wait_ms, 1.0)));
+                    continue;
+                }
+            }
             segs = S->sched.next_launch();
         } else if (!admitted.empty()) {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -241,8 +290,11 @@ void worker_main(hss_server* S) {
                 }
                 if (segs.size() > 1) ++S->st.shared_launches;
             }
+            const double t_plan = S->now();
             int rc = launch_plan(c, S->bufs[L.buf], pr.data(), (uint32_t)pr.size());
             if (rc == HSS_OK) rc = launch_enqueue(c, S->bufs[L.buf], S->cnt, c->stream);
+            L.work = S->bufs[L.buf].plan.est_work;
+            clk.planned(S->now() - t_plan);
             if (rc != HSS_OK) {
                 fail_all(S, rc, hss_last_error());
                 (void)hipStreamSynchronize(c->stream);
@@ -270,6 +322,7 @@ void worker_main(hss_server* S) {
                 }
                 rc = launch_enqueue(c, B, S->cnt, c->stream);
                 if (rc == HSS_OK) {
+                    L.t_start = -1;
                     inflight.push_back(std::move(L));
                     continue;
                 }
@@ -281,6 +334,7 @@ void worker_main(hss_server* S) {
             return;
         }
         free_mask |= 1u << L.buf;
+        clk.finished(L.work, (double)R.setup_ms + R.cells_ms + R.exact_ms);
         rc = read_back(S, L.segs);
         if (rc != HSS_OK) {
             fail_all(S, rc, hss_last_error());

@@ -245,7 +245,8 @@ def test_prefix_with_out_of_order_launches(hc):
 
 def test_equal_share_keeps_short_requests_moving(hc):
     """A long request does not hold a short one back: both get work in the
-    first launch, the short one finishes in it."""
+    first launch, the short one finishes in it, and because the short one
+    could finish within a quarter of the budget the launch takes half of it."""
     s = Sched(hc, 8, 10000, 100.0)
     s.enqueue(1, 5000)
     s.enqueue(2, 5)
@@ -255,8 +256,34 @@ def test_equal_share_keeps_short_requests_moving(hc):
     segs = s.next()
     assert (2, 0, 5) in segs
     first = [x for x in segs if x[0] == 1][0]
-    assert first[2] - first[1] == 95  # its share (50) plus the rest of the budget
+    assert first[2] - first[1] == 45  # its share of the half budget (25) plus the rest (20)
     assert s.finished(segs) == [2]
+    s.retire(2)
+    # only the long request left: full launches again
+    segs = s.next()
+    assert segs == [(1, 45, 145)]
+    s.close()
+
+
+def test_rest_of_budget_to_least_work_left(hc):
+    """After the equal shares, the rest of a launch's budget goes to the
+    requests with the least work left, not to the oldest."""
+    s = Sched(hc, 8, 100000, 120.0)
+    s.enqueue(1, 10000)   # oldest, most work
+    s.enqueue(2, 200)
+    s.enqueue(3, 10)
+    assert s.admit() == [1, 2, 3]
+    for rid, n in [(1, 10000), (2, 200), (3, 10)]:
+        s.set_costs(rid, [1.0] * n)
+    segs = {x[0]: x[2] - x[1] for x in s.next()}
+    # 3 can finish within a quarter of 120: half budget 60, shares of 20; 3
+    # takes its 10 and the 10 left over go to 2 (180 left) before 1
+    assert segs == {1: 20, 2: 30, 3: 10}
+    # no short request left: full budget 120 in shares of 60
+    s.finished([(3, 0, 10)])
+    s.retire(3)
+    segs = {x[0]: x[2] - x[1] for x in s.next()}
+    assert segs == {1: 60, 2: 60}
     s.close()
 
 
