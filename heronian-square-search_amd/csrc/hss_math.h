@@ -196,14 +196,16 @@ HSS_HD inline bool root_candidate_ok(double zz, double r) { return root_candidat
 // tol as in root_candidate_ok (device); the host form ignores it
 HSS_HD inline bool root_near_integer(double zz, double tol) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    // zz^(1/4) from two hardware reciprocal square roots (t = zz^-1/2, then
-    // t^-1/2), each within 2^-22 relative on a pessimistic reading of their
-    // 2^29-ulp bound, so r starts within 1.5 * 2^-22; one root4_refine step
-    // leaves 2.5 (1.5 * 2^-22)^2 + 3 * 2^-44 < 2^-40.8, i.e. < 4e-5 absolute
-    // for n <= 2^26, and rint(r) is the only candidate n.  (Even with the
-    // hardware bound 2^6 times worse the step stays within 0.2 absolute.)
-    const double t = __builtin_amdgcn_rsq(zz);
-    return root_candidate_ok(zz, root4_refine(zz, t, __builtin_amdgcn_rsq(t)), tol);
+    // zz^(1/4) from two FP32 hardware reciprocal square roots (t = zz^-1/2 of
+    // zz rounded to FP32, then t^-1/2; zz <= 2^104 is in FP32 range): the
+    // rounding (2^-24) and the instruction's 1-ulp bound (2^-23) keep each
+    // within 2^-22 relative, so r starts within 1.5 * 2^-22; one root4_refine
+    // step in FP64 leaves 2.5 (1.5 * 2^-22)^2 + 3 * 2^-44 < 2^-40.8, i.e.
+    // < 4e-5 absolute for n <= 2^26, and rint(r) is the only candidate n.
+    // (Even with errors 2^6 times worse the step stays within 0.2 absolute.)
+    // The FP32 forms issue faster than the FP64 ones.
+    const float t32 = __builtin_amdgcn_rsqf((float)zz);
+    return root_candidate_ok(zz, root4_refine(zz, (double)t32, (double)__builtin_amdgcn_rsqf(t32)), tol);
 #else
     (void)tol;
     double r = __builtin_sqrt(__builtin_sqrt(zz));
