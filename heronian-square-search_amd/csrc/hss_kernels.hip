@@ -712,7 +712,7 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
     __shared__ QEntry s_q[WAVES][QCAP];
     __shared__ uint32_t s_mark[WAVES][65];
     __shared__ KCtx s_kctx[WAVES][64];
-    __shared__ uint32_t s_ncls[WAVES][16];
+    __shared__ uint32_t s_ncls[WAVES][32];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     RootQueue rq{s_q[wave], 0, 0, 0};
     uint32_t* mark = s_mark[wave];
@@ -730,8 +730,11 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
         const bool one = __builtin_amdgcn_readfirstlane(tile_one_term(L.q, T.k0, S.l4_d, S.l2_d, S.n4_d) ? 1 : 0) != 0;
         uint32_t cnt = 0;
         uint64_t leaves = 0, pairs = 0, evald = 0;
-        // for the count of leaf orientations the rows test: ncls[x + 4 msk] = the
-        // non-central entries u with (class of u ^ x) & msk == 0
+        // for the count of leaf orientations the rows test: with n(x, msk) the
+        // non-central entries u with (class of u ^ x) & msk == 0, ncls[x + 4 msk +
+        // 16 both] = n(x, msk) + (both ? n(x ^ 3, msk) : 0) -- the rows of a
+        // partner entry of class x test n(x, msk) entries in orientation 1 and,
+        // when both orientations can count (both), n(x ^ 3, msk) in orientation 2
         if (L.H > 1) {
             uint64_t h01 = 0, h23 = 0;  // class counts, 32 bits each
             for (uint32_t i = (uint32_t)lane; i + 1 < L.H; i += 64) {
@@ -741,14 +744,19 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
             }
             h01 = wave_sum64(h01);
             h23 = wave_sum64(h23);
-            const uint32_t x = (uint32_t)lane & 3u, msk = ((uint32_t)lane >> 2) & 3u;
-            uint32_t n = 0;
-            n += (x & msk) == 0u ? (uint32_t)h01 : 0u;
-            n += ((1u ^ x) & msk) == 0u ? (uint32_t)(h01 >> 32) : 0u;
-            n += ((2u ^ x) & msk) == 0u ? (uint32_t)h23 : 0u;
-            n += ((3u ^ x) & msk) == 0u ? (uint32_t)(h23 >> 32) : 0u;
+            const uint32_t msk = ((uint32_t)lane >> 2) & 3u;
+            const auto n_of = [&](uint32_t x) {
+                uint32_t n = 0;
+                n += (x & msk) == 0u ? (uint32_t)h01 : 0u;
+                n += ((1u ^ x) & msk) == 0u ? (uint32_t)(h01 >> 32) : 0u;
+                n += ((2u ^ x) & msk) == 0u ? (uint32_t)h23 : 0u;
+                n += ((3u ^ x) & msk) == 0u ? (uint32_t)(h23 >> 32) : 0u;
+                return n;
+            };
+            const uint32_t x = (uint32_t)lane & 3u;
+            const uint32_t n = n_of(x) + (((uint32_t)lane & 16u) ? n_of(x ^ 3u) : 0u);
             __builtin_amdgcn_wave_barrier();
-            if (lane < 16) ncls[lane] = n;
+            if (lane < 32) ncls[lane] = n;
             __builtin_amdgcn_wave_barrier();
         }
         // the pair's band and area info once per k (a k's half-set entries share them)
@@ -812,11 +820,11 @@ __global__ __launch_bounds__(WG, 6) void cell_kernel(HSS_CELL_ARGS) {
                 }
             };
             const double cu = lf[L.H - 1].re;  // the central entry is real
-            // leaf orientations the rows test: entries matching t1, then t2
-            if (L.H > 1 && act) {
-                const uint32_t nrow = (t1 == 0u ? ncls[v.cls + 4u * cmsk] : 0u) +
-                                      (t2 != 4u ? ncls[(v.cls ^ 3u) + 4u * cmsk] : 0u);
-                evald += (uint64_t)nrow << (v.vc ? 0 : 1);
+            // leaf orientations the rows test: entries matching t1, then t2 (a
+            // forced run has cmsk = 0: all H - 1 rows in its one orientation)
+            if (L.H > 1) {
+                const uint32_t nrow = ncls[v.cls | cmsk << 2 | (t2 == cm && cmsk != 0u ? 16u : 0u)];
+                evald += act ? (uint64_t)(nrow << (v.vc ? 0 : 1)) : 0ull;
             }
             bool cev;
             if (one) {
