@@ -396,8 +396,16 @@ bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P
                         hi = mid - 1;
                 }
                 uint32_t kend = std::max(lo, std::min(kmax, k0 + 127u));
-                const double cells = (double)L.H * (double)entries(off, k0, kend);
-                double cost = cells + (double)entries(off, k0, kend) * tune.plain_partner_cost;
+                const uint64_t ne = entries(off, k0, kend);
+                const double cells = (double)L.H * (double)ne;
+                // plain mode runs every row of the left set for each step of 64
+                // partner entries, and each chunk of 64 partner indices of one
+                // parity takes a step of its own: a tile of a few entries over a
+                // large left set costs its steps (H rows each), not its cells
+                const uint64_t per_parity = ((uint64_t)kend - k0 + 2) / 2;
+                const uint64_t chunks = (P.pos[L.pos].l_even ? 1u : 2u) * ((per_parity + 63) / 64);
+                const uint64_t steps = std::max((ne + 63) / 64, std::min(chunks, ne));
+                double cost = (double)L.H * 64.0 * (double)steps + (double)ne * tune.plain_partner_cost;
                 bool win = false;
                 if (eligible[li]) {
                     const double f = tile_window_fraction(P.pos[L.pos].l, P.pos[L.pos].limit, L.q, k0, kend, 3);
