@@ -37,13 +37,15 @@ inline int weight_class(uint32_t H) {
     return c < 0 ? 0 : (c >= WEIGHT_CLASSES ? WEIGHT_CLASSES - 1 : c);
 }
 
-// Defaults: hss_autotune on MI355X, calibration roots drawn like the uneven
-// LIMIT 1e7 workload with another seed.
+// Defaults on MI355X, from calibration roots drawn like the uneven LIMIT 1e7
+// workload with another seed: tiles_per_wave and win_max_frac by
+// hss_autotune, the window cost weights by a sweep (window item 0.5 and
+// partner entry 8 against 1.4 and 4: 4.5 % less kernel time on those roots).
 struct Tuning {
     uint32_t tiles_per_wave = 32;  // tile size: est. work / (waves * tiles_per_wave)
     double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 0.8, 1.0, 0.5, 0.6, 1.0, 0.8, 0.5};
-    double win_item_cost = 1.4;   // cost of a window item relative to a plain leaf (tile order)
-    double win_partner_cost = 4;  // per partner entry of a window tile, in plain-cell units
+    double win_item_cost = 0.5;   // cost of a window item relative to a plain leaf (tile order)
+    double win_partner_cost = 8;  // per partner entry of a window tile, in plain-cell units
     double plain_partner_cost = 2.5;  // per partner entry of a plain tile, in plain-cell units
 };
 
