@@ -909,20 +909,18 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         const bool liveu = wi < nunits;
         const uint32_t wq = liveu ? wi - first : 0u;
         const WinVp& Q = *(const WinVp*)((const char*)vps + __umul24(o, (uint32_t)sizeof(WinVp)));
-        uint32_t uc[8];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t w = Q.uc[i];
-            uc[2 * i] = w & 0xFFFFu;
-            uc[2 * i + 1] = w >> 16;
-        }
-        const uint32_t pr = uc[0];
+        const uint32_t pr = Q.uc[0] & 0xFFFFu;
         // the unit's range pc: the number of ranges p >= 1 whose units-before
-        // uc[p] <= wq (uc never decreases), compares without branches; ucp =
-        // uc[pc] read back by its halfword address (range 0 starts at unit 0)
-        uint32_t pc = 0;
+        // uc[p] <= wq (uc never decreases), two halfwords at a time: every
+        // count is below 2^15 (8 ranges of at most 2048 units) and so is pr,
+        // so (2^15 + wq) - uc[p] in each half keeps its guard bit 15 exactly
+        // when uc[p] <= wq and never borrows from the next half; ucp = uc[pc]
+        // read back by its halfword address (range 0 starts at unit 0)
+        static_assert(8 * ((WIN_MAX_SORT + WIN_UNIT - 1) / WIN_UNIT) < 0x8000u, "unit counts below 2^15");
+        const uint32_t wg = (wq | wq << 16) | 0x80008000u;
+        uint32_t pc = __popc((wg - Q.uc[0]) & 0x80000000u);  // its low half is pr
 #pragma unroll
-        for (int p = 1; p < 8; ++p) pc += uc[p] <= wq ? 1u : 0u;
+        for (int i = 1; i < 4; ++i) pc += __popc((wg - Q.uc[i]) & 0x80008000u);
         const uint32_t ucp = pc ? (uint32_t)*(const uint16_t*)((const char*)Q.uc + 2u * pc) : 0u;
         const uint32_t se = Q.se[pc];
         const uint32_t ui0 = (se & 0xFFFFu) + (wq - ucp) * WIN_UNIT;
