@@ -276,6 +276,24 @@ __global__ __launch_bounds__(256) void k_build_left(const BatchHeader* hdr, cons
     }
 }
 
+// Angle modulo pi and class of the entry of rank t of a left set, from its
+// digits: the value of digit d of prime p_j is pi_j^d conj(pi_j)^(4 E_j - d),
+// of angle (2d - 4E_j) theta_j, and the fixed part is a positive integer, so
+// the entry's angle is sum_j (2 d_j - 4 E_j) theta_j (theta_j modulo pi is
+// enough: the multipliers are even).  FP64 error below 1e-12 rad (at most
+// MAX_SPLIT terms with multipliers below 64 and angles below pi); the window
+// budget (hss_math.h) allows 1e-7 for the FP32 rounding of the key.
+__device__ inline float left_angle_class(const LeftBuild& b, uint32_t t, const double* th, uint32_t& cls) {
+    uint32_t dig[MAX_SPLIT];
+    decode_rank(t, b.k, b.E, dig);
+    double a = 0.0;
+    for (uint32_t j = 0; j < b.k; ++j) a = __builtin_fma((double)(2 * (int)dig[j] - 4 * (int)b.E[j]), th[j], a);
+    a -= PI_D * __builtin_floor(a * (1.0 / PI_D));
+    a = a < 0.0 ? a + PI_D : (a >= PI_D ? a - PI_D : a);
+    cls = digits_class(b.k, b.p, dig);
+    return (float)a;
+}
+
 // Bucket of a sorted left set's entry: its area group's slice of the WIN_NB
 // buckets (WIN_NB >> group bits each), then its angle theta modulo pi.
 __device__ inline uint32_t sort_bucket(uint32_t group, uint32_t gbits, float key) {
@@ -302,6 +320,7 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
     __shared__ uint32_t start[WIN_NB + 1];    // bucket counts, then bucket starts
     __shared__ uint32_t cursor[WIN_NB];
     __shared__ uint32_t part[256];
+    __shared__ double th[MAX_SPLIT];          // angles of the left set's split primes, modulo pi
     const uint32_t tid = threadIdx.x;
     const uint32_t nsorted = hdr->nsorted;
     for (uint32_t bi = blockIdx.x; bi < nsorted; bi += gridDim.x) {  // uniform per block
@@ -312,12 +331,17 @@ __global__ __launch_bounds__(256) void k_left_sort(const BatchHeader* hdr, const
         const uint32_t gbits = __popc(kmask);
         const uint32_t n = L.H - 1;  // sortable entries (L.H <= WIN_MAX_SORT + 1, planned)
         for (uint32_t bk = tid; bk <= WIN_NB; bk += blockDim.x) start[bk] = 0;
+        // theta_j from the value of digit 2 E_j + 1, p^(2E_j - 1) pi_j^2: half its angle
+        if (tid < b.k) {
+            const uint32_t at = vbase[b.p[tid]] + exp_offset(b.E[tid]) + 2 * b.E[tid] + 1;
+            th[tid] = 0.5 * atan2(i128_to_double(vim[at]), i128_to_double(vre[at]));
+        }
         __syncthreads();
         for (uint32_t t = tid; t < n; t += blockDim.x) {
-            const GInt z = left_value(b, t, vbase, vre, vim);
-            const float k = (float)theta_of(i128_to_double(z.re), i128_to_double(z.im));
+            uint32_t cls;
+            const float k = left_angle_class(b, t, th, cls);
             key[t] = k;
-            atomicAdd(&start[sort_bucket(area_group(left_class(b, t), kmask), gbits, k)], 1u);
+            atomicAdd(&start[sort_bucket(area_group(cls, kmask), gbits, k)], 1u);
         }
         __syncthreads();
         // exclusive scan of the WIN_NB counts, WIN_NB / 256 per thread

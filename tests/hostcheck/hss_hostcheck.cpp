@@ -181,6 +181,9 @@ int hc_search_mixed(uint64_t nmax, const uint64_t* roots, const uint64_t* limits
         for (uint32_t t = 0; t < L.H; ++t) perm[t] = t;
         if (L.bt != NO_TABLE) {
             // k_left_sort: (area group, theta) keys, FP32 angles, group slices of the buckets
+            // (the device sums the primes' angles over the digits instead of taking the
+            // product's angle: the same to 1e-12 rad before the FP32 rounding; either
+            // order gives exact counts, the window covers the key's error)
             const uint32_t n = L.H - 1, kmask = left_kmask(L), gbits = __builtin_popcount(kmask);
             const uint32_t nbg = WIN_NB >> gbits;
             std::vector<float> key(n);
