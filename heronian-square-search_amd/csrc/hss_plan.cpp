@@ -368,9 +368,16 @@ bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P
     const uint32_t tpw = tune.tiles_per_wave ? tune.tiles_per_wave : 8;
     double target = total / (double)(waves ? waves * tpw : tpw);
     target = std::min(std::max(target, 8192.0), 4194304.0);
+    // heaviest tiles first, per mode: a counting sort on the cost quantised to
+    // 1/8 octaves (load balance needs the order only roughly; linear time)
+    constexpr int KEYS = 8 * 64;
+    auto key = [](double c) {
+        const int k = c > 1.0 ? (int)(8.0 * std::log2(c)) : 0;
+        return (uint32_t)(KEYS - 1 - std::min(k, KEYS - 1));  // descending cost
+    };
     struct Cost {
-        double c;
-        uint32_t i;
+        uint32_t key;  // sort key of the tile's cost
+        uint32_t i;    // the tile in its chunk
         bool win;
     };
     struct TileChunk {
@@ -417,42 +424,27 @@ bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P
                     }
                 }
                 C.work += cost;
-                C.costs.push_back({cost, (uint32_t)C.tiles.size(), win});
+                C.costs.push_back({key(cost), (uint32_t)C.tiles.size(), win});
                 C.tiles.push_back(TileDesc{li, k0, kend, 0});
                 if (kend == kmax) break;
                 k0 = kend + 1;
             }
         }
     });
-    std::vector<Cost> order;
-    std::vector<TileDesc> all;
-    for (TileChunk& C : tc) {
-        for (Cost& x : C.costs) {
-            x.i += (uint32_t)all.size();
-            order.push_back(x);
-        }
-        all.insert(all.end(), C.tiles.begin(), C.tiles.end());
+    // the counting sort per mode, placed straight from the chunks (within a
+    // key: chunk order, then tile order)
+    std::vector<uint32_t> start[2] = {std::vector<uint32_t>(KEYS + 1, 0), std::vector<uint32_t>(KEYS + 1, 0)};
+    for (const TileChunk& C : tc) {
+        for (const Cost& x : C.costs) ++start[x.win][x.key + 1];
         P.win_cells += C.win_cells;
         P.est_work += C.work;
     }
-    tc.clear();
-    // heaviest tiles first, per mode: a counting sort on the cost quantised to
-    // 1/8 octaves (load balance needs the order only roughly; linear time)
-    constexpr int KEYS = 8 * 64;
-    auto key = [](double c) {
-        const int k = c > 1.0 ? (int)(8.0 * std::log2(c)) : 0;
-        return KEYS - 1 - std::min(k, KEYS - 1);  // descending cost
-    };
-    std::vector<uint32_t> start(KEYS + 1, 0);
-    for (const Cost& c : order) ++start[key(c.c) + 1];
-    for (int k = 0; k < KEYS; ++k) start[k + 1] += start[k];
-    std::vector<uint32_t> sorted(order.size());
-    for (uint32_t i = 0; i < order.size(); ++i) sorted[start[key(order[i].c)]++] = i;
-    size_t nwin = 0;
-    for (const Cost& c : order) nwin += c.win ? 1 : 0;
-    P.wtiles.reserve(nwin);
-    P.tiles.reserve(order.size() - nwin);
-    for (uint32_t i : sorted) (order[i].win ? P.wtiles : P.tiles).push_back(all[order[i].i]);
+    for (int m = 0; m < 2; ++m)
+        for (int k = 0; k < KEYS; ++k) start[m][k + 1] += start[m][k];
+    P.tiles.resize(start[0][KEYS]);
+    P.wtiles.resize(start[1][KEYS]);
+    for (const TileChunk& C : tc)
+        for (const Cost& x : C.costs) (x.win ? P.wtiles : P.tiles)[start[x.win][x.key]++] = C.tiles[x.i];
     return true;
 }
 
