@@ -293,7 +293,12 @@ bool plan_lefts(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
             });
         }
     });
-    for (PosChunk& C : pc) {
+    // merge: each chunk's left sets go to their slice, entry offsets shifted
+    // by the entries of the chunks before it
+    std::vector<uint32_t> first(pchunks + 1, 0);
+    std::vector<uint64_t> eoff(pchunks, 0);
+    for (uint32_t c = 0; c < pchunks; ++c) {
+        const PosChunk& C = pc[c];
         if (!C.err.empty()) {
             err = C.err;
             return false;
@@ -302,15 +307,28 @@ bool plan_lefts(const PosReq* reqs, uint32_t n_roots, const std::vector<uint32_t
             err = "batch too large: split it";
             return false;
         }
-        for (LeftDesc& L : C.lefts) L.off += (uint32_t)P.left_entries;
+        eoff[c] = P.left_entries;
         P.left_entries += C.entries;
         P.est_cells += C.cells;
-        P.lefts.insert(P.lefts.end(), C.lefts.begin(), C.lefts.end());
-        P.builds.insert(P.builds.end(), C.builds.begin(), C.builds.end());
-        P.kmin.insert(P.kmin.end(), C.kmin.begin(), C.kmin.end());
-        P.kmax.insert(P.kmax.end(), C.kmax.begin(), C.kmax.end());
+        first[c + 1] = first[c] + (uint32_t)C.lefts.size();
     }
-    pc.clear();
+    P.lefts.resize(first[pchunks]);
+    P.builds.resize(first[pchunks]);
+    P.kmin.resize(first[pchunks]);
+    P.kmax.resize(first[pchunks]);
+    parallel_chunks(pchunks, pchunks, [&](uint32_t b, uint32_t e, uint32_t) {
+        for (uint32_t c = b; c < e; ++c) {
+            const PosChunk& C = pc[c];
+            for (size_t i = 0; i < C.lefts.size(); ++i) {
+                LeftDesc L = C.lefts[i];
+                L.off += (uint32_t)eoff[c];
+                P.lefts[first[c] + i] = L;
+            }
+            std::copy(C.builds.begin(), C.builds.end(), P.builds.begin() + first[c]);
+            std::copy(C.kmin.begin(), C.kmin.end(), P.kmin.begin() + first[c]);
+            std::copy(C.kmax.begin(), C.kmax.end(), P.kmax.begin() + first[c]);
+        }
+    });
     // Window mode per left set: eligible by size, then per tile by the
     // window fraction of its partner range.  Work is estimated in plain-cell
     // units per partner entry: a plain tile H + plain_partner_cost (the cells
