@@ -531,6 +531,43 @@ int hc_split_by_work(uint64_t nmax, uint64_t limit, uint64_t first, uint64_t las
     return split_by_work(limit, first, last, parts, g_tables.small, g_tables.off.data(), nmax, bounds, err) ? 0 : 1;
 }
 
+// The planner's tiles of a batch of positions at one limit, planned for
+// `waves` resident waves under the current tuning: the tiles of every left
+// set (in either list) must cover its partner range [kmin, kmax] exactly once.
+// out: [0] plain tiles, [1] window tiles, [2] left sets, [3] left sets whose
+// tiles leave a gap, overlap or run past the range, [4] tiles with k1 < k0.
+// 0 on success (the plan was made).
+int hc_plan_cover(uint64_t nmax, uint64_t limit, const uint64_t* roots, uint32_t n_roots, uint64_t waves,
+                  uint64_t* out) {
+    if (g_tables.n != nmax) build(g_tables, nmax);
+    std::vector<PosReq> reqs(n_roots);
+    for (uint32_t s = 0; s < n_roots; ++s) reqs[s] = PosReq{roots[s], limit, s};
+    Plan P;
+    std::string err;
+    if (!plan_batch(reqs.data(), n_roots, g_tables.small, g_tables.off.data(), nmax, waves, g_tune, P, err)) return 1;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by_left(P.lefts.size());
+    uint64_t reversed = 0;
+    for (const std::vector<TileDesc>* list : {&P.tiles, &P.wtiles})
+        for (const TileDesc& t : *list) {
+            reversed += t.k1 < t.k0 ? 1 : 0;
+            by_left[t.left].push_back({t.k0, t.k1});
+        }
+    uint64_t bad = 0;
+    for (size_t li = 0; li < P.lefts.size(); ++li) {
+        std::vector<std::pair<uint32_t, uint32_t>>& r = by_left[li];
+        std::sort(r.begin(), r.end());
+        bool ok = !r.empty() && r.front().first == P.kmin[li] && r.back().second == P.kmax[li];
+        for (size_t i = 1; ok && i < r.size(); ++i) ok = r[i].first == r[i - 1].second + 1;
+        bad += ok ? 0 : 1;
+    }
+    out[0] = P.tiles.size();
+    out[1] = P.wtiles.size();
+    out[2] = P.lefts.size();
+    out[3] = bad;
+    out[4] = reversed;
+    return 0;
+}
+
 // Planner work estimate of one position (cells), or -1.
 double hc_estimate(uint64_t nmax, uint64_t l, uint64_t limit) {
     if (g_tables.n != nmax) build(g_tables, nmax);

@@ -197,6 +197,35 @@ def test_split_by_work(hc, limit, first, last, parts):
         assert part <= total / parts + max(work) * (1 + 1e-12)
 
 
+def plan_cover(hc, nmax, limit, roots, waves):
+    n = len(roots)
+    out = (ctypes.c_uint64 * 5)()
+    rc = hc.hc_plan_cover(ctypes.c_uint64(nmax), ctypes.c_uint64(limit), (ctypes.c_uint64 * n)(*roots),
+                          ctypes.c_uint32(n), ctypes.c_uint64(waves), out)
+    assert rc == 0
+    return list(out)
+
+
+@pytest.mark.parametrize("waves", [64, 4096, 1 << 20])
+@pytest.mark.parametrize("tuning", ["default", "window_everywhere", "window_off"])
+def test_plan_tiles_cover_each_left_set_once(hc, waves, tuning, request):
+    """The planner's tiles (both lists, any order) partition every left set's
+    partner range [kmin, kmax]: no gap, no overlap, nothing past the range --
+    for coarse and fine tiles (few or many resident waves) and for every
+    window-mode setting.  A gap would lose counts and an overlap double them,
+    which only the device parity tests would otherwise show."""
+    if tuning != "default":
+        request.getfixturevalue(tuning)
+    roots = list(range(1, 3001, 7)) + [11789, 65000, 69997, 70000] + [2 * 3 * 5 * 13 * 17, 5 ** 4 * 13 * 2]
+    plain, win, lefts, bad, rev = plan_cover(hc, 70000, 70000, roots, waves)
+    assert lefts > len(roots) and plain + win >= lefts
+    assert bad == 0 and rev == 0
+    if tuning == "window_off":
+        assert win == 0
+    if tuning == "window_everywhere":
+        assert win > 0
+
+
 def test_split_by_work_rejects(hc):
     assert split(hc, 70000, 70000, 10, 9, 2)[0] != 0
     assert split(hc, 70000, 70000, 1, 10, 0)[0] != 0
