@@ -651,9 +651,9 @@ __device__ inline bool central_win(const CellParams& P, const TileCtx& tc, bool 
     cnt += (uint32_t)__popc(f) + ((!und && real2 && h2n >= 0.0) ? 1u : 0u);
     const double a = __builtin_fabs(D);
     bool pass = false;
-    if (__builtin_amdgcn_ballot_w64(f & 1u)) pass = (f & 1u) && root_near_integer(__builtin_fma(-2.0, a, b.lm));
+    if (__builtin_amdgcn_ballot_w64(f & 1u)) pass = ((f & 1u) != 0u) & root_near_integer(__builtin_fma(-2.0, a, b.lm));
     if (!ONE && __builtin_amdgcn_ballot_w64(f & 2u))
-        pass |= (f & 2u) && root_near_integer(__builtin_fma(2.0, a, b.lm), tc.tol2);
+        pass |= ((f & 2u) != 0u) & root_near_integer(__builtin_fma(2.0, a, b.lm), tc.tol2);
     if (__builtin_amdgcn_ballot_w64(pass)) push_global(P, pass, DefItem{tc.left, ui, v.vidx, v.kj});
     return (ok && !v.vc) || real2;  // the leaf had an orientation to test
 }
@@ -992,7 +992,9 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         for (uint32_t s = 0; s < WIN_UNIT; ++s) {
             wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 1u) != 0u));
             if constexpr (!ONE) wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 2u) != 0u));
-            pass[s] = (F[s] & 1u) && root_near_integer(__builtin_fma(-2.0, AX[s], b.lm), tol1);
+            // both operands evaluated (&, not &&): nearly every evaluated leaf
+            // has a counted term, so a per-leaf branch would only add mask work
+            pass[s] = ((F[s] & 1u) != 0u) & root_near_integer(__builtin_fma(-2.0, AX[s], b.lm), tol1);
         }
         if constexpr (!ONE) {
             bool any2 = false;
@@ -1001,7 +1003,7 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
             if (__builtin_amdgcn_ballot_w64(any2)) {
 #pragma unroll
                 for (uint32_t s = 0; s < WIN_UNIT; ++s)
-                    pass[s] |= (F[s] & 2u) && root_near_integer(__builtin_fma(2.0, AX[s], b.lm), tc.tol2);
+                    pass[s] |= ((F[s] & 2u) != 0u) & root_near_integer(__builtin_fma(2.0, AX[s], b.lm), tc.tol2);
             }
         }
 #pragma unroll
