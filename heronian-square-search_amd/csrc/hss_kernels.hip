@@ -967,14 +967,16 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         // entries past uend are masked by live
         const LeftFast* up = (const LeftFast*)((const char*)lf + __umul24(ui0, (uint32_t)sizeof(LeftFast)));
         double AX[WIN_UNIT];
-        uint32_t F[WIN_UNIT];
+        bool F1[WIN_UNIT], F2[WIN_UNIT];  // counted terms 1 and 2 (o1_decide's bits, kept as lane masks)
         bool U[WIN_UNIT];
 #pragma unroll
         for (uint32_t s = 0; s < WIN_UNIT; ++s) {
             const LeftFast u = up[s];
             const double X = __builtin_fma(u.re, ce, -(u.im * de));
             AX[s] = __builtin_fabs(X);
-            F[s] = o1_decide<ONE>(s < nlive, X, b, U[s]);
+            const uint32_t f = o1_decide<ONE>(s < nlive, X, b, U[s]);
+            F1[s] = (f & 1u) != 0u;
+            F2[s] = (f & 2u) != 0u;
         }
         bool uany = false;
 #pragma unroll
@@ -990,20 +992,20 @@ __device__ inline void win_units(const CellParams& P, const TileCtx& tc, const L
         const double tol1 = root_tol1(b.lm);
 #pragma unroll
         for (uint32_t s = 0; s < WIN_UNIT; ++s) {
-            wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 1u) != 0u));
-            if constexpr (!ONE) wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64((F[s] & 2u) != 0u));
+            wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(F1[s]));
+            if constexpr (!ONE) wcnt += (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(F2[s]));
             // both operands evaluated (&, not &&): nearly every evaluated leaf
             // has a counted term, so a per-leaf branch would only add mask work
-            pass[s] = ((F[s] & 1u) != 0u) & root_near_integer(__builtin_fma(-2.0, AX[s], b.lm), tol1);
+            pass[s] = F1[s] & root_near_integer(__builtin_fma(-2.0, AX[s], b.lm), tol1);
         }
         if constexpr (!ONE) {
             bool any2 = false;
 #pragma unroll
-            for (uint32_t s = 0; s < WIN_UNIT; ++s) any2 |= (F[s] & 2u) != 0u;
+            for (uint32_t s = 0; s < WIN_UNIT; ++s) any2 |= F2[s];
             if (__builtin_amdgcn_ballot_w64(any2)) {
 #pragma unroll
                 for (uint32_t s = 0; s < WIN_UNIT; ++s)
-                    pass[s] |= ((F[s] & 2u) != 0u) & root_near_integer(__builtin_fma(2.0, AX[s], b.lm), tc.tol2);
+                    pass[s] |= F2[s] & root_near_integer(__builtin_fma(2.0, AX[s], b.lm), tc.tol2);
             }
         }
 #pragma unroll
