@@ -40,10 +40,12 @@ inline int weight_class(uint32_t H) {
 // Defaults on MI355X, from calibration roots drawn like the uneven LIMIT 1e7
 // workload with another seed: the window cost weights by a sweep (window
 // item 0.5 and partner entry 8 against 1.4 and 4: 4.5 % less kernel time on
-// those roots), then tiles_per_wave and win_max_frac by hss_autotune on the
-// whole calibration batch under those weights (3.5 % less again).
+// those roots), then win_max_frac by hss_autotune on the whole calibration
+// batch under those weights (3.5 % less again), then tiles_per_wave 48 by a
+// sweep of the whole search's time (1.3 % less than 32: finer tiles cost
+// planning time, which the device's left-set stage now hides).
 struct Tuning {
-    uint32_t tiles_per_wave = 32;  // tile size: est. work / (waves * tiles_per_wave)
+    uint32_t tiles_per_wave = 48;  // tile size: est. work / (waves * tiles_per_wave)
     double win_max_frac[WEIGHT_CLASSES] = {0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.8};
     double win_item_cost = 0.5;   // cost of a window item relative to a plain leaf (tile order)
     double win_partner_cost = 8;  // per partner entry of a window tile, in plain-cell units

@@ -402,7 +402,7 @@ extern "C" int hss_autotune(hss_ctx* c, uint64_t limit, const uint64_t* roots, u
                 best = t;
             }
         }
-    for (uint32_t tpw : {4u, 8u, 16u, 32u}) {
+    for (uint32_t tpw : {8u, 16u, 32u, 48u, 64u}) {
         if (tpw == best.tiles_per_wave) continue;
         Tuning t = best;
         t.tiles_per_wave = tpw;

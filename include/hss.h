@@ -173,7 +173,7 @@ int hss_ctx_set_tuning(hss_ctx* ctx, const hss_tuning* in);
 /*
  * Autotuner: searches win_max_frac per weight class (coordinate descent over
  * {0, 0.05, 0.1, 0.2, 0.3, 0.45, 0.6, 0.8, 1}, classes in order of their share
- * of the work) and then tiles_per_wave over {4, 8, 16, 32}, timing the device
+ * of the work) and then tiles_per_wave over {8, 16, 32, 48, 64}, timing the device
  * part of hss_search (HIP events, best of `repeats`) on the calibration roots
  * at `limit`.  Every setting must give the same candidate total (else
  * HSS_EDEVICE).  The best tuning is applied to ctx and copied to *out
