@@ -411,8 +411,20 @@ bool plan_tiles(const uint32_t* off, uint64_t waves, const Tuning& tune, Plan& P
             uint32_t k0 = P.kmin[li], kmax = P.kmax[li];
             uint64_t budget = (uint64_t)std::max(target / ecost[li], 256.0);
             while (k0 <= kmax) {
-                // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices
+                // largest kend in [k0, kmax] with entries(k0, kend) <= budget, at least 128 indices;
+                // entries(k0, .) grows with kend and a tile spans a small part
+                // of [k0, kmax], so probe k0 + 127, k0 + 255, ... (cache lines
+                // near k0) until one is over budget, then bisect the last gap
                 uint32_t lo = k0, hi = kmax;
+                for (uint64_t step = 128;; step *= 2) {
+                    const uint64_t probe = (uint64_t)k0 + step - 1;
+                    if (probe >= hi) break;
+                    if (entries(off, k0, (uint32_t)probe) > budget) {
+                        hi = (uint32_t)probe - 1;
+                        break;
+                    }
+                    lo = (uint32_t)probe;
+                }
                 while (lo < hi) {
                     uint32_t mid = lo + (hi - lo + 1) / 2;
                     if (entries(off, k0, mid) <= budget)
